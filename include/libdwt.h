@@ -1,0 +1,161 @@
+/*
+ * libdwt drop-in C interface, served by hand-written HIP kernels on an
+ * AMD Instinct MI355X (gfx950).
+ *
+ * Every function of the first part keeps the signature and semantics of the
+ * reference libdwt function it replaces (the citation after each group is the
+ * reference declaration in src/libdwt.h):
+ *   - strides are in bytes and may be odd, unaligned or negative;
+ *     stride_x is the distance between rows, stride_y between columns;
+ *   - the 2D drivers work in place on the outer frame size_o x size_o, the
+ *     nested image size_i x size_i is what gets transformed; *j_max_ptr is
+ *     clamped to the achievable depth and written back (forward);
+ *   - calls are synchronous: the result is in memory when the call returns;
+ *   - results are bit-identical to the reference (FP32 "_s" and FP64 "_d").
+ * Pointers may be device memory (hipMalloc), managed memory, pinned host
+ * memory (hipHostMalloc) or ordinary host memory; the last is staged through
+ * device memory.  The `tmp` argument of the 1D functions is accepted and not
+ * used (device scratch is managed internally).
+ * On a HIP error or invalid argument a message is printed to stderr and the
+ * process aborts, as the reference does on its assertion and
+ * dwt_util_abort() paths; the dwt_gpu_* extensions return an error code
+ * instead.
+ */
+#ifndef LIBDWT_H
+#define LIBDWT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- one level, contiguous (deprecated forms: L and H concatenated) ------
+ * reference: src/libdwt.h:29-120 (dwt_cdf97_f_d .. dwt_cdf53_i_s) */
+void dwt_cdf97_f_d(const double *src, double *dst, double *tmp, int N);
+void dwt_cdf53_f_d(const double *src, double *dst, double *tmp, int N);
+void dwt_cdf97_f_s(const float *src, float *dst, float *tmp, int N);
+void dwt_cdf53_f_s(const float *src, float *dst, float *tmp, int N);
+void dwt_cdf97_i_d(const double *src, double *dst, double *tmp, int N);
+void dwt_cdf53_i_d(const double *src, double *dst, double *tmp, int N);
+void dwt_cdf97_i_s(const float *src, float *dst, float *tmp, int N);
+void dwt_cdf53_i_s(const float *src, float *dst, float *tmp, int N);
+
+/* ---- one level, separate L/H channels -------------------------------------
+ * reference: src/libdwt.h:134-198 (forward _ex), 250-312 (inverse _ex) */
+void dwt_cdf97_f_ex_d(const double *src, double *dst_l, double *dst_h, double *tmp, int N);
+void dwt_cdf53_f_ex_d(const double *src, double *dst_l, double *dst_h, double *tmp, int N);
+void dwt_cdf97_f_ex_s(const float *src, float *dst_l, float *dst_h, float *tmp, int N);
+void dwt_cdf53_f_ex_s(const float *src, float *dst_l, float *dst_h, float *tmp, int N);
+void dwt_cdf97_i_ex_d(const double *src_l, const double *src_h, double *dst, double *tmp, int N);
+void dwt_cdf53_i_ex_d(const double *src_l, const double *src_h, double *dst, double *tmp, int N);
+void dwt_cdf97_i_ex_s(const float *src_l, const float *src_h, float *dst, float *tmp, int N);
+void dwt_cdf53_i_ex_s(const float *src_l, const float *src_h, float *dst, float *tmp, int N);
+
+/* ---- one level on a strided line ------------------------------------------
+ * reference: src/libdwt.h:205-240 (forward), 319-348 (inverse) */
+void dwt_cdf97_f_ex_stride_d(const double *src, double *dst_l, double *dst_h, double *tmp, int N, int stride);
+void dwt_cdf53_f_ex_stride_d(const double *src, double *dst_l, double *dst_h, double *tmp, int N, int stride);
+void dwt_cdf97_f_ex_stride_s(const float *src, float *dst_l, float *dst_h, float *tmp, int N, int stride);
+void dwt_cdf53_f_ex_stride_s(const float *src, float *dst_l, float *dst_h, float *tmp, int N, int stride);
+void dwt_cdf97_i_ex_stride_d(const double *src_l, const double *src_h, double *dst, double *tmp, int N, int stride);
+void dwt_cdf53_i_ex_stride_d(const double *src_l, const double *src_h, double *dst, double *tmp, int N, int stride);
+void dwt_cdf97_i_ex_stride_s(const float *src_l, const float *src_h, float *dst, float *tmp, int N, int stride);
+void dwt_cdf53_i_ex_stride_s(const float *src_l, const float *src_h, float *dst, float *tmp, int N, int stride);
+
+/* ---- zero padding ----------------------------------------------------------
+ * reference: src/libdwt.h:355-448 */
+void dwt_zero_padding_f_d(double *dst_l, double *dst_h, int N, int N_dst_L, int N_dst_H);
+void dwt_zero_padding_f_s(float *dst_l, float *dst_h, int N, int N_dst_L, int N_dst_H);
+void dwt_zero_padding_f_stride_d(double *dst_l, double *dst_h, int N, int N_dst_L, int N_dst_H, int stride);
+void dwt_zero_padding_f_stride_s(float *dst_l, float *dst_h, int N, int N_dst_L, int N_dst_H, int stride);
+void dwt_zero_padding_i_d(double *dst_l, int N, int N_dst);
+void dwt_zero_padding_i_s(float *dst_l, int N, int N_dst);
+void dwt_zero_padding_i_stride_d(double *dst_l, int N, int N_dst, int stride);
+void dwt_zero_padding_i_stride_s(float *dst_l, int N, int N_dst, int stride);
+
+/* ---- 2D multilevel, in place ----------------------------------------------
+ * reference: src/libdwt.h:455-520 (forward), 527-592 (inverse) */
+void dwt_cdf97_2f_d(void *ptr, int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
+                    int size_i_big_x, int size_i_big_y, int *j_max_ptr, int decompose_one,
+                    int zero_padding);
+void dwt_cdf53_2f_d(void *ptr, int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
+                    int size_i_big_x, int size_i_big_y, int *j_max_ptr, int decompose_one,
+                    int zero_padding);
+void dwt_cdf97_2f_s(void *ptr, int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
+                    int size_i_big_x, int size_i_big_y, int *j_max_ptr, int decompose_one,
+                    int zero_padding);
+void dwt_cdf53_2f_s(void *ptr, int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
+                    int size_i_big_x, int size_i_big_y, int *j_max_ptr, int decompose_one,
+                    int zero_padding);
+void dwt_cdf97_2i_d(void *ptr, int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
+                    int size_i_big_x, int size_i_big_y, int j_max, int decompose_one,
+                    int zero_padding);
+void dwt_cdf53_2i_d(void *ptr, int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
+                    int size_i_big_x, int size_i_big_y, int j_max, int decompose_one,
+                    int zero_padding);
+void dwt_cdf97_2i_s(void *ptr, int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
+                    int size_i_big_x, int size_i_big_y, int j_max, int decompose_one,
+                    int zero_padding);
+void dwt_cdf53_2i_s(void *ptr, int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
+                    int size_i_big_x, int size_i_big_y, int j_max, int decompose_one,
+                    int zero_padding);
+
+/* ===========================================================================
+ * Extensions (no counterpart in the reference).  They return 0 on success or
+ * a negative DWT_GPU_E* code; dwt_gpu_last_error() describes the last failure
+ * of the calling thread.
+ * ======================================================================== */
+enum dwt_gpu_wavelet { DWT_GPU_CDF97 = 0, DWT_GPU_CDF53 = 1 };
+enum dwt_gpu_direction { DWT_GPU_FORWARD = 0, DWT_GPU_INVERSE = 1 };
+enum dwt_gpu_precision { DWT_GPU_F32 = 0, DWT_GPU_F64 = 1 };
+enum dwt_gpu_status {
+	DWT_GPU_OK = 0,
+	DWT_GPU_EINVAL = -1,   /* bad argument */
+	DWT_GPU_EMEMORY = -2,  /* pointer not device-accessible / allocation failed */
+	DWT_GPU_EHIP = -3      /* HIP runtime error */
+};
+
+/* Initialise the library on the current device (optional; done on first use). */
+int dwt_gpu_init(void);
+/* Text of the calling thread's last error ("" when none). */
+const char *dwt_gpu_last_error(void);
+/* Version string of this library. */
+const char *dwt_gpu_version(void);
+
+/* 2D multilevel transform of device-accessible memory, enqueued on `stream`
+ * (a hipStream_t; NULL = the null stream) without synchronising.  Arguments
+ * as dwt_cdf97_2f_s / dwt_cdf97_2i_s; *j is read (inverse) or clamped and
+ * written (forward).  Calls on one stream are ordered; scratch memory is kept
+ * per (host thread, stream), so concurrent calls must use distinct streams or
+ * distinct threads. */
+int dwt_gpu_2d_async(int wavelet, int direction, int precision, void *ptr, int stride_x,
+                     int stride_y, int size_o_big_x, int size_o_big_y, int size_i_big_x,
+                     int size_i_big_y, int *j, int decompose_one, int zero_padding,
+                     void *stream);
+
+/* Per-launch timing with HIP events recorded on the launch stream around each
+ * kernel of the packed path.  kind: 0 forward level kernel, 1 inverse level
+ * kernel, 2 level-0 detail copy; level: the level index (-1 = all).
+ * profile_read synchronises on the recorded events. */
+int dwt_gpu_profile_enable(int on);
+int dwt_gpu_profile_read(int kind, int level, long long *launches, double *total_ms);
+int dwt_gpu_profile_reset(void);
+
+/* Tile shape of the packed level kernels: a wavefront walks seg_rows rows
+ * (even, 2..4096) of a 128-column strip, prefetching prefetch_pairs (4 or 8)
+ * row pairs.  Shapes are kept per direction (forward 1 / inverse 0) and level
+ * height class ceil(log2(height)); seg_rows = 0 restores the built-in choice.
+ * dwt_gpu_tile_get reports the shape a level of that size would use. */
+int dwt_gpu_tile_set(int forward, int height_class, int seg_rows, int prefetch_pairs);
+int dwt_gpu_tile_get(int forward, int width, int height, int wavelet, int *seg_rows,
+                     int *prefetch_pairs);
+
+/* Bytes of device scratch the calling thread currently holds for `stream`. */
+long long dwt_gpu_scratch_bytes(void *stream);
+/* Release the calling thread's scratch for `stream` (synchronises it). */
+int dwt_gpu_release_scratch(void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIBDWT_H */

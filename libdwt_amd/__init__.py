@@ -1,0 +1,178 @@
+"""Python mirror of libdwt's C interface over the MI355X HIP library.
+
+The product is ``libdwt_gpu.so`` (C ABI, see include/libdwt.h).  This module
+binds it with ctypes under the reference's own names and argument order
+(src/libdwt.h), and adds helpers that run the transforms on torch device
+tensors.  Loading fails loudly when the library has not been built: there is
+no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdwt_gpu.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "libdwt.h")
+
+CDF97, CDF53 = 0, 1
+FORWARD, INVERSE = 0, 1
+F32, F64 = 0, 1
+WAVELET_IDS = {"cdf97": CDF97, "cdf53": CDF53}
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def header_functions() -> list[str]:
+    """Names of every function declared in include/libdwt.h."""
+    with open(HEADER) as fh:
+        text = fh.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.findall(r"^\s*(?:const\s+)?(?:void|int|long long|char)\s*\*?\s*(\w+)\s*\(", text,
+                      flags=re.M)
+
+
+def lib() -> C.CDLL:
+    """The loaded HIP library (raises LibraryMissing when not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryMissing(
+                f"{LIB_PATH} is missing: build it with `python -m libdwt_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, i = C.c_void_p, C.c_int
+        for w in ("cdf97", "cdf53"):
+            for s in ("s", "d"):
+                f = getattr(L, f"dwt_{w}_2f_{s}")
+                f.argtypes = [vp, i, i, i, i, i, i, C.POINTER(i), i, i]
+                f.restype = None
+                f = getattr(L, f"dwt_{w}_2i_{s}")
+                f.argtypes = [vp, i, i, i, i, i, i, i, i, i]
+                f.restype = None
+                for d in ("f", "i"):
+                    f = getattr(L, f"dwt_{w}_{d}_ex_stride_{s}")
+                    f.argtypes = [vp, vp, vp, vp, i, i]
+                    f.restype = None
+                    f = getattr(L, f"dwt_{w}_{d}_ex_{s}")
+                    f.argtypes = [vp, vp, vp, vp, i]
+                    f.restype = None
+                    f = getattr(L, f"dwt_{w}_{d}_{s}")
+                    f.argtypes = [vp, vp, vp, i]
+                    f.restype = None
+        for s in ("s", "d"):
+            getattr(L, f"dwt_zero_padding_f_stride_{s}").argtypes = [vp, vp, i, i, i, i]
+            getattr(L, f"dwt_zero_padding_f_{s}").argtypes = [vp, vp, i, i, i]
+            getattr(L, f"dwt_zero_padding_i_stride_{s}").argtypes = [vp, i, i, i]
+            getattr(L, f"dwt_zero_padding_i_{s}").argtypes = [vp, i, i]
+        L.dwt_gpu_init.restype = i
+        L.dwt_gpu_last_error.restype = C.c_char_p
+        L.dwt_gpu_version.restype = C.c_char_p
+        L.dwt_gpu_2d_async.argtypes = [i, i, i, vp, i, i, i, i, i, i, C.POINTER(i), i, i, vp]
+        L.dwt_gpu_2d_async.restype = i
+        L.dwt_gpu_profile_enable.argtypes = [i]
+        L.dwt_gpu_profile_enable.restype = i
+        L.dwt_gpu_profile_read.argtypes = [i, i, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
+        L.dwt_gpu_profile_read.restype = i
+        L.dwt_gpu_profile_reset.restype = i
+        L.dwt_gpu_tile_set.argtypes = [i, i, i, i]
+        L.dwt_gpu_tile_set.restype = i
+        L.dwt_gpu_tile_get.argtypes = [i, i, i, i, C.POINTER(i), C.POINTER(i)]
+        L.dwt_gpu_tile_get.restype = i
+        L.dwt_gpu_scratch_bytes.argtypes = [vp]
+        L.dwt_gpu_scratch_bytes.restype = C.c_longlong
+        L.dwt_gpu_release_scratch.argtypes = [vp]
+        L.dwt_gpu_release_scratch.restype = i
+        _lib = L
+    return _lib
+
+
+class DwtError(RuntimeError):
+    pass
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise DwtError(f"libdwt_gpu error {rc}: {lib().dwt_gpu_last_error().decode()}")
+
+
+# -- reference-named calls (addresses are plain integers) ----------------------
+
+def cdf_2f(wavelet: str, prec: str, ptr: int, stride_x: int, stride_y: int, size_o_x: int,
+           size_o_y: int, size_i_x: int, size_i_y: int, j_max: int, decompose_one: int = 0,
+           zero_padding: int = 0) -> int:
+    """dwt_{wavelet}_2f_{prec}; returns the achieved number of levels."""
+    j = C.c_int(j_max)
+    getattr(lib(), f"dwt_{wavelet}_2f_{prec}")(ptr, stride_x, stride_y, size_o_x, size_o_y,
+                                               size_i_x, size_i_y, C.byref(j), decompose_one,
+                                               zero_padding)
+    return j.value
+
+
+def cdf_2i(wavelet: str, prec: str, ptr: int, stride_x: int, stride_y: int, size_o_x: int,
+           size_o_y: int, size_i_x: int, size_i_y: int, j_max: int, decompose_one: int = 0,
+           zero_padding: int = 0) -> None:
+    getattr(lib(), f"dwt_{wavelet}_2i_{prec}")(ptr, stride_x, stride_y, size_o_x, size_o_y,
+                                               size_i_x, size_i_y, j_max, decompose_one,
+                                               zero_padding)
+
+
+def transform_2d_async(wavelet: str, forward: bool, ptr: int, stride_x: int, stride_y: int,
+                       size_o_x: int, size_o_y: int, size_i_x: int | None = None,
+                       size_i_y: int | None = None, j: int = -1, decompose_one: int = 0,
+                       zero_padding: int = 0, f64: bool = False, stream: int = 0) -> int:
+    """dwt_gpu_2d_async on device memory; returns the (clamped) level count."""
+    jj = C.c_int(j)
+    rc = lib().dwt_gpu_2d_async(WAVELET_IDS[wavelet], FORWARD if forward else INVERSE,
+                                F64 if f64 else F32, ptr, stride_x, stride_y, size_o_x, size_o_y,
+                                size_o_x if size_i_x is None else size_i_x,
+                                size_o_y if size_i_y is None else size_i_y, C.byref(jj),
+                                decompose_one, zero_padding, stream)
+    _check(rc)
+    return jj.value
+
+
+def profile_enable(on: bool) -> None:
+    lib().dwt_gpu_profile_enable(1 if on else 0)
+
+
+def profile_reset() -> None:
+    lib().dwt_gpu_profile_reset()
+
+
+def profile_read(kind: int, level: int = -1) -> tuple[int, float]:
+    """(launches, total milliseconds) of the recorded launches of one kind
+    (0 forward level, 1 inverse level, 2 detail copy) at one level (-1 all)."""
+    n = C.c_longlong(0)
+    ms = C.c_double(0)
+    _check(lib().dwt_gpu_profile_read(kind, level, C.byref(n), C.byref(ms)))
+    return n.value, ms.value
+
+
+# -- torch helpers ----------------------------------------------------------------
+
+def fwd2(t, wavelet: str = "cdf97", j: int = -1, decompose_one: int = 0) -> int:
+    """In-place forward 2D transform of a contiguous (H, W) float32/float64 CUDA tensor."""
+    return _run2(t, wavelet, True, j, decompose_one)
+
+
+def inv2(t, wavelet: str = "cdf97", j: int = -1, decompose_one: int = 0) -> None:
+    _run2(t, wavelet, False, j, decompose_one)
+
+
+def _run2(t, wavelet, forward, j, d1):
+    import torch
+    if not t.is_cuda or t.dim() != 2 or t.dtype not in (torch.float32, torch.float64):
+        raise ValueError("expects a 2D float32/float64 CUDA tensor")
+    if t.stride(1) != 1:
+        raise ValueError("expects unit element stride along rows")
+    es = t.element_size()
+    H, W = t.shape
+    stream = torch.cuda.current_stream(t.device).cuda_stream
+    jj = transform_2d_async(wavelet, forward, t.data_ptr(), t.stride(0) * es, es, W, H, W, H, j,
+                            d1, 0, t.dtype == torch.float64, stream)
+    return jj

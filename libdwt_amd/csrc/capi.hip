@@ -1,0 +1,836 @@
+// C-ABI layer: libdwt's drop-in entry points and the dwt_gpu_* extensions.
+//
+// Each entry point classifies its pointer(s), runs the transform with the
+// device kernels on the calling thread's stream, and synchronises before it
+// returns (the reference's calls are synchronous).  Host (pageable) memory is
+// staged: the frame or line is gathered into packed device memory, transformed
+// there, and scattered back.  There is no CPU compute path: if the device work
+// fails the call aborts (drop-in) or returns an error (extensions).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "dwt_common.h"
+#include "kernels.h"
+#include "../../include/libdwt.h"
+
+namespace dwt {
+
+// ---- constants (src/libdwt.c:541-573) -----------------------------------------
+namespace k {
+static const float P1_s = 1.58613434342059;
+static const float U1_s = -0.0529801185729;
+static const float P2_s = -0.8829110755309;
+static const float U2_s = 0.4435068520439;
+static const float K_s = 1.1496043988602;
+static const float S2_s = 1 / 1.1496043988602;
+static const double P1_d = 1.58613434342059;
+static const double U1_d = -0.0529801185729;
+static const double P2_d = -0.8829110755309;
+static const double U2_d = 0.4435068520439;
+static const double K_d = 1.1496043988602;
+static const double S2_d = 1 / 1.1496043988602;
+static const float P53_s = 0.5, U53_s = 0.25;
+static const float S153_s = 1.41421356237309504880, S253_s = 0.70710678118654752440;
+static const double P53_d = 0.5, U53_d = 0.25;
+static const double S153_d = 1.41421356237309504880, S253_d = 0.70710678118654752440;
+}  // namespace k
+
+template <>
+Coeffs<float> make_coeffs<float>(Wavelet w, bool fwd) {
+  Coeffs<float> c{};
+  // volatile keeps the float quotient 1/K a run-time float division, the
+  // value the reference's lifting path multiplies by (src/libdwt.c:1604)
+  volatile float kk = k::K_s;
+  const float invK = 1.0f / kk;
+  if (w == CDF97) {
+    c.nsteps = 4;
+    if (fwd) {  // src/libdwt.c:5542
+      c.c[0] = -k::P1_s; c.c[1] = k::U1_s; c.c[2] = -k::P2_s; c.c[3] = k::U2_s;
+      c.se = k::K_s; c.so = invK; c.one = k::K_s; c.first_parity = 1;
+    } else {    // src/libdwt.c:5811, 5793
+      c.c[0] = -k::U2_s; c.c[1] = k::P2_s; c.c[2] = -k::U1_s; c.c[3] = k::P1_s;
+      c.se = invK; c.so = k::K_s; c.one = k::S2_s; c.first_parity = 0;
+    }
+  } else {
+    c.nsteps = 2;
+    if (fwd) {  // src/libdwt.c:5576-5598
+      c.c[0] = -k::P53_s; c.c[1] = k::U53_s;
+      c.se = k::S153_s; c.so = k::S253_s; c.one = k::S153_s; c.first_parity = 1;
+    } else {    // src/libdwt.c:5843-5866
+      c.c[0] = -k::U53_s; c.c[1] = k::P53_s;
+      c.se = k::S253_s; c.so = k::S153_s; c.one = k::S253_s; c.first_parity = 0;
+    }
+  }
+  return c;
+}
+
+template <>
+Coeffs<double> make_coeffs<double>(Wavelet w, bool fwd) {
+  Coeffs<double> c{};
+  if (w == CDF97) {
+    c.nsteps = 4;
+    if (fwd) {  // src/libdwt.c:1467-1525
+      c.c[0] = -k::P1_d; c.c[1] = k::U1_d; c.c[2] = -k::P2_d; c.c[3] = k::U2_d;
+      c.se = k::K_d; c.so = k::S2_d; c.one = k::K_d; c.first_parity = 1;
+    } else {    // src/libdwt.c:5673-5731
+      c.c[0] = -k::U2_d; c.c[1] = k::P2_d; c.c[2] = -k::U1_d; c.c[3] = k::P1_d;
+      c.se = k::S2_d; c.so = k::K_d; c.one = k::S2_d; c.first_parity = 0;
+    }
+  } else {
+    c.nsteps = 2;
+    if (fwd) {  // src/libdwt.c:1527-1571
+      c.c[0] = -k::P53_d; c.c[1] = k::U53_d;
+      c.se = k::S153_d; c.so = k::S253_d; c.one = k::S153_d; c.first_parity = 1;
+    } else {    // src/libdwt.c:5733-5777
+      c.c[0] = -k::U53_d; c.c[1] = k::P53_d;
+      c.se = k::S253_d; c.so = k::S153_d; c.one = k::S253_d; c.first_parity = 0;
+    }
+  }
+  return c;
+}
+
+// ---- errors -------------------------------------------------------------------
+static thread_local std::string g_err;
+
+struct Failure {
+  int code;
+};
+
+static void fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  throw Failure{code};
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) fail(DWT_GPU_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// hipGetLastError() returns the last error of *any* earlier HIP call of this
+// thread and keeps it until read, so a failure left behind by other code would
+// be reported as ours.  LAUNCH() discards such a stale error first and then
+// checks only what the enclosed launches returned.
+static void check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) fail(DWT_GPU_EHIP, "kernel launch: %s", hipGetErrorString(e));
+}
+#define LAUNCH(...)            \
+  do {                         \
+    (void)hipGetLastError();   \
+    __VA_ARGS__;               \
+    check_launch();            \
+  } while (0)
+
+// ---- device scratch, one set per (host thread, stream, device) ----------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+};
+
+struct Scratch {
+  // 0: LL of even levels, 1: LL of odd levels, 2: level-0 details,
+  // 3: line temp, 4: host staging
+  DevBuf buf[5];
+  hipStream_t stream = nullptr;
+  // side stream for the level-0 detail copies, which touch memory disjoint
+  // from what the other levels read and write, so they overlap those levels
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  void ensure_side() {
+    if (side) return;
+    HIPCHK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+  }
+  void* get(int i, size_t bytes) {
+    DevBuf& b = buf[i];
+    if (b.n < bytes) {
+      if (b.p) {
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipFree(b.p));
+        b.p = nullptr;
+        b.n = 0;
+      }
+      size_t want = bytes + (bytes >> 3);
+      want = (want + 4095) & ~size_t(4095);
+      if (hipMalloc(&b.p, want) != hipSuccess) {
+        hipGetLastError();
+        b.p = nullptr;
+        fail(DWT_GPU_EMEMORY, "hipMalloc of %zu bytes of scratch failed", want);
+      }
+      b.n = want;
+    }
+    return b.p;
+  }
+  long long bytes() const {
+    long long s = 0;
+    for (const auto& b : buf) s += (long long)b.n;
+    return s;
+  }
+  void release() {
+    for (auto& b : buf) {
+      if (b.p) hipFree(b.p);
+      b = DevBuf{};
+    }
+  }
+};
+
+static std::mutex g_scratch_mu;
+static std::map<std::tuple<std::thread::id, hipStream_t, int>, Scratch*>* g_scratch =
+    new std::map<std::tuple<std::thread::id, hipStream_t, int>, Scratch*>();
+
+static Scratch& scratch_for(hipStream_t st) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto key = std::make_tuple(std::this_thread::get_id(), st, dev);
+  auto it = g_scratch->find(key);
+  if (it != g_scratch->end()) return *it->second;
+  Scratch* s = new Scratch();
+  s->stream = st;
+  (*g_scratch)[key] = s;
+  return *s;
+}
+
+// ---- optional per-launch timing (HIP events on the launch stream) ---------------
+// Kinds: 0 forward level kernel, 1 inverse level kernel, 2 detail copy,
+// 3 generic line pass.  Level is the level index the launch belongs to.
+struct ProfRec {
+  int kind, level;
+  hipEvent_t a, b;
+};
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<ProfRec>* g_prof = new std::vector<ProfRec>();
+
+struct ProfScope {
+  bool on = false;
+  ProfRec r{};
+  hipStream_t st;
+  ProfScope(int kind, int level, hipStream_t s) : st(s) {
+    {
+      std::lock_guard<std::mutex> lk(g_prof_mu);
+      on = g_prof_on;
+    }
+    if (!on) return;
+    r.kind = kind;
+    r.level = level;
+    HIPCHK(hipEventCreate(&r.a));
+    HIPCHK(hipEventCreate(&r.b));
+    HIPCHK(hipEventRecord(r.a, st));
+  }
+  ~ProfScope() {
+    if (!on) return;
+    hipEventRecord(r.b, st);
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof->push_back(r);
+  }
+};
+
+static std::once_flag g_init_once;
+static hipError_t g_init_status = hipSuccess;
+static void init_once() {
+  std::call_once(g_init_once, [] { g_init_status = configure_line_kernels(); });
+  if (g_init_status != hipSuccess)
+    fail(DWT_GPU_EHIP, "setting the line kernels' LDS limit: %s", hipGetErrorString(g_init_status));
+}
+
+static size_t pad_pitch(size_t w) { return (w + 63) & ~size_t(63); }
+
+// ---- generic path: per-level line passes in place on any byte strides --------
+template <typename T>
+static void generic_fwd_level(char* ptr, long long sx, long long sy, int Wo, int Ho, int Wi, int Hi,
+                              int j, int zp, const Coeffs<T>& cf, Scratch& sc, hipStream_t st) {
+  const int Wos = ceil_div_pow2(Wo, j), Hos = ceil_div_pow2(Ho, j);
+  const int Wod = ceil_div_pow2(Wo, j + 1), Hod = ceil_div_pow2(Ho, j + 1);
+  const int Wis = ceil_div_pow2(Wi, j), His = ceil_div_pow2(Hi, j);
+  const int lim = line_lds_limit_bytes() / (int)sizeof(T);
+  const int gblocks = 1024;
+  // rows (src/libdwt.c:6354-6362), then columns (6367-6375)
+  T* gt = Wis > lim ? (T*)sc.get(3, sizeof(T) * (size_t)Wis * gblocks) : nullptr;
+  LAUNCH(launch_line_pass<T>(true, ptr, ptr, ptr + (long long)Wod * sy, sx, sy, Hos, Wis, cf, gt, gblocks, st));
+  gt = His > lim ? (T*)sc.get(3, sizeof(T) * (size_t)His * gblocks) : nullptr;
+  LAUNCH(launch_line_pass<T>(true, ptr, ptr, ptr + (long long)Hod * sx, sy, sx, Wos, His, cf, gt, gblocks, st));
+  if (zp) {  // src/libdwt.c:6377-6397
+    const int zx[4] = {(Wis + 1) / 2, Wod, Wod + Wis / 2, Wos};
+    const int zy[4] = {(His + 1) / 2, Hod, Hod + His / 2, Hos};
+    LAUNCH(launch_zero_pad<T>(ptr, sx, sy, Wos, Hos, zx, zy, st));
+  }
+}
+
+template <typename T>
+static void generic_inv_level(char* ptr, long long sx, long long sy, int Wo, int Ho, int Wi, int Hi,
+                              int j, int zp, const Coeffs<T>& cf, Scratch& sc, hipStream_t st) {
+  const int Wos = ceil_div_pow2(Wo, j), Hos = ceil_div_pow2(Ho, j);
+  const int Wod = ceil_div_pow2(Wo, j - 1), Hod = ceil_div_pow2(Ho, j - 1);
+  const int Wid = ceil_div_pow2(Wi, j - 1), Hid = ceil_div_pow2(Hi, j - 1);
+  const int lim = line_lds_limit_bytes() / (int)sizeof(T);
+  const int gblocks = 1024;
+  // rows (src/libdwt.c:6706-6714), then columns (6719-6727)
+  T* gt = Wid > lim ? (T*)sc.get(3, sizeof(T) * (size_t)Wid * gblocks) : nullptr;
+  LAUNCH(launch_line_pass<T>(false, ptr, ptr + (long long)Wos * sy, ptr, sx, sy, Hod, Wid, cf, gt, gblocks, st));
+  gt = Hid > lim ? (T*)sc.get(3, sizeof(T) * (size_t)Hid * gblocks) : nullptr;
+  LAUNCH(launch_line_pass<T>(false, ptr, ptr + (long long)Hos * sx, ptr, sy, sx, Wod, Hid, cf, gt, gblocks, st));
+  if (zp) {  // src/libdwt.c:6729-6745
+    const int zx[4] = {Wid, Wod, 0, 0};
+    const int zy[4] = {Hid, Hod, 0, 0};
+    LAUNCH(launch_zero_pad<T>(ptr, sx, sy, Wod, Hod, zx, zy, st));
+  }
+}
+
+// ---- packed fast path -----------------------------------------------------------
+// A: W x H, row pitch p elements, levels [0, L) forward.  Level 0 reads A and
+// writes its detail subbands to scratch D (they overlap A's unread input);
+// level j >= 1 reads LL_j from ping-pong scratch P[j % 2] and writes its
+// details straight into A (that region of A was consumed by level 0); the
+// last level writes its LL into A; finally D's details are copied into A.
+template <typename T>
+static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, Scratch& sc,
+                     hipStream_t st) {
+  const int W1 = ceil_div_pow2(W, 1), H1 = ceil_div_pow2(H, 1);
+  const int W2 = ceil_div_pow2(W, 2), H2 = ceil_div_pow2(H, 2);
+  const long long Dp = (long long)pad_pitch(W);
+  T* D = (T*)sc.get(2, sizeof(T) * (size_t)Dp * H);
+  T* P[2] = {nullptr, nullptr};
+  long long Pp[2] = {(long long)pad_pitch(W2), (long long)pad_pitch(W1)};
+  if (L >= 2) P[1] = (T*)sc.get(1, sizeof(T) * (size_t)Pp[1] * H1);
+  if (L >= 3) P[0] = (T*)sc.get(0, sizeof(T) * (size_t)Pp[0] * H2);
+  sc.ensure_side();
+  for (int j = 0; j < L; ++j) {
+    const int Wj = ceil_div_pow2(W, j), Hj = ceil_div_pow2(H, j);
+    const T* src = j == 0 ? A : P[j % 2];
+    const long long sp = j == 0 ? p : Pp[j % 2];
+    T* ll;
+    long long llp;
+    if (j == L - 1) {
+      ll = j == 0 ? D : A;
+      llp = j == 0 ? Dp : p;
+    } else {
+      ll = P[(j + 1) % 2];
+      llp = Pp[(j + 1) % 2];
+    }
+    T* det = j == 0 ? D : A;
+    const long long dp = j == 0 ? Dp : p;
+    {
+      ProfScope ps(0, j, st);
+      LAUNCH(launch_fwd_level<T>(src, sp, Wj, Hj, ll, llp, det, dp, cf, st));
+    }
+    if (j == 0) {
+      // details of level 0 (and its LL when it is the only level) into A, on
+      // the side stream while levels >= 1 run on st
+      HIPCHK(hipEventRecord(sc.fork, st));
+      HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
+      ProfScope ps(2, 0, sc.side);
+      LAUNCH(launch_copy_rect<T>(D + W1, Dp, A + W1, p, W - W1, H1, sc.side);
+             launch_copy_rect<T>(D + (long long)H1 * Dp, Dp, A + (long long)H1 * p, p, W, H - H1, sc.side);
+             if (L == 1) launch_copy_rect<T>(D, Dp, A, p, W1, H1, sc.side));
+    }
+  }
+  HIPCHK(hipEventRecord(sc.join, sc.side));
+  HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
+}
+
+// Inverse of fast_fwd: LL_L is in A.  Level j >= 2 writes LL_{j-1} to P;
+// level 1 writes the image into A, so the level-0 details (and LL_1 when
+// L == 1) are first copied out of A into D.
+template <typename T>
+static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, Scratch& sc,
+                     hipStream_t st) {
+  const int W1 = ceil_div_pow2(W, 1), H1 = ceil_div_pow2(H, 1);
+  const int W2 = ceil_div_pow2(W, 2), H2 = ceil_div_pow2(H, 2);
+  const long long Dp = (long long)pad_pitch(W);
+  T* D = (T*)sc.get(2, sizeof(T) * (size_t)Dp * H);
+  T* P[2] = {nullptr, nullptr};
+  long long Pp[2] = {(long long)pad_pitch(W2), (long long)pad_pitch(W1)};
+  if (L >= 2) P[1] = (T*)sc.get(1, sizeof(T) * (size_t)Pp[1] * H1);
+  if (L >= 3) P[0] = (T*)sc.get(0, sizeof(T) * (size_t)Pp[0] * H2);
+  // copy the level-0 details (and LL_1 when it is the only level) out of A on
+  // the side stream while levels L..2 run; level 1 waits for it
+  sc.ensure_side();
+  HIPCHK(hipEventRecord(sc.fork, st));
+  HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
+  {
+    ProfScope ps(2, 0, sc.side);
+    LAUNCH(launch_copy_rect<T>(A + W1, p, D + W1, Dp, W - W1, H1, sc.side);
+           launch_copy_rect<T>(A + (long long)H1 * p, p, D + (long long)H1 * Dp, Dp, W, H - H1, sc.side);
+           if (L == 1) launch_copy_rect<T>(A, p, D, Dp, W1, H1, sc.side));
+  }
+  HIPCHK(hipEventRecord(sc.join, sc.side));
+  for (int j = L; j >= 1; --j) {
+    const int Wf = ceil_div_pow2(W, j - 1), Hf = ceil_div_pow2(H, j - 1);
+    const T* ll;
+    long long llp;
+    if (j == L) {
+      ll = L == 1 ? D : A;
+      llp = L == 1 ? Dp : p;
+    } else {
+      ll = P[j % 2];
+      llp = Pp[j % 2];
+    }
+    const T* det = j == 1 ? D : A;
+    const long long dp = j == 1 ? Dp : p;
+    T* dst = j == 1 ? A : P[(j - 1) % 2];
+    const long long dstp = j == 1 ? p : Pp[(j - 1) % 2];
+    if (j == 1) HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
+    {
+      ProfScope ps(1, j - 1, st);
+      LAUNCH(launch_inv_level<T>(ll, llp, det, dp, dst, dstp, Wf, Hf, cf, st));
+    }
+  }
+}
+
+// ---- 2D driver on device-accessible memory -------------------------------------
+template <typename T>
+static bool packed_layout(const char* ptr, long long sx, long long sy, int Wo, int Ho, int Wi,
+                          int Hi) {
+  return sy == (long long)sizeof(T) && sx > 0 && sx % (long long)sizeof(T) == 0 &&
+         sx >= (long long)Wo * (long long)sizeof(T) &&
+         (reinterpret_cast<uintptr_t>(ptr) % sizeof(T)) == 0 && Wo == Wi && Ho == Hi;
+}
+
+template <typename T>
+static int fast_levels(int W, int H, int J) {
+  int jf = 0;
+  while (jf < J && ceil_div_pow2(W, jf) >= 2 && ceil_div_pow2(H, jf) >= 2) ++jf;
+  return jf;
+}
+
+static int level_limit(int Wo, int Ho, int d1) {
+  const int mn = Wo < Ho ? Wo : Ho, mx = Wo < Ho ? Ho : Wo;
+  return ceil_log2(d1 ? mx : mn);  // src/libdwt.c:6324
+}
+
+template <typename T>
+static void fwd2_device(char* ptr, long long sx, long long sy, int Wo, int Ho, int Wi, int Hi,
+                        int* jp, int d1, int zp, Wavelet w, hipStream_t st) {
+  init_once();
+  const int jl = level_limit(Wo, Ho, d1);
+  if (*jp < 0 || *jp > jl) *jp = jl;  // src/libdwt.c:6326-6329
+  const int J = *jp;
+  if (J == 0 || Wo <= 0 || Ho <= 0) return;
+  Scratch& sc = scratch_for(st);
+  const Coeffs<T> cf = make_coeffs<T>(w, true);
+  int jf = 0;
+  if (packed_layout<T>(ptr, sx, sy, Wo, Ho, Wi, Hi)) jf = fast_levels<T>(Wo, Ho, J);
+  if (jf > 0) fast_fwd<T>(reinterpret_cast<T*>(ptr), sx / (long long)sizeof(T), Wo, Ho, jf, cf, sc, st);
+  for (int j = jf; j < J; ++j) generic_fwd_level<T>(ptr, sx, sy, Wo, Ho, Wi, Hi, j, zp, cf, sc, st);
+}
+
+template <typename T>
+static void inv2_device(char* ptr, long long sx, long long sy, int Wo, int Ho, int Wi, int Hi,
+                        int jmax, int d1, int zp, Wavelet w, hipStream_t st) {
+  init_once();
+  int J = level_limit(Wo, Ho, d1);
+  if (jmax >= 0 && jmax < J) J = jmax;  // src/libdwt.c:6678-6681
+  if (J == 0 || Wo <= 0 || Ho <= 0) return;
+  Scratch& sc = scratch_for(st);
+  const Coeffs<T> cf = make_coeffs<T>(w, false);
+  int jf = 0;
+  if (packed_layout<T>(ptr, sx, sy, Wo, Ho, Wi, Hi)) jf = fast_levels<T>(Wo, Ho, J);
+  for (int j = J; j > jf; --j) generic_inv_level<T>(ptr, sx, sy, Wo, Ho, Wi, Hi, j, zp, cf, sc, st);
+  if (jf > 0) fast_inv<T>(reinterpret_cast<T*>(ptr), sx / (long long)sizeof(T), Wo, Ho, jf, cf, sc, st);
+}
+
+// ---- pointer classification and staging --------------------------------------------
+// Returns the device-usable address of p, or nullptr for pageable host memory.
+static void* device_address(const void* p) {
+  if (!p) return nullptr;
+  hipPointerAttribute_t a;
+  hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    hipGetLastError();
+    return nullptr;
+  }
+  switch (a.type) {
+    case hipMemoryTypeDevice:
+    case hipMemoryTypeManaged:
+    case hipMemoryTypeUnified:
+      return const_cast<void*>(p);
+    case hipMemoryTypeHost:
+      if (a.devicePointer) {
+        return static_cast<char*>(a.devicePointer) +
+               (static_cast<const char*>(p) - static_cast<const char*>(a.hostPointer));
+      }
+      return nullptr;
+    default:
+      return nullptr;
+  }
+}
+
+// Gather the outer frame of a host image into a packed host vector and back.
+template <typename T>
+static void host_gather(std::vector<T>& v, const char* ptr, long long sx, long long sy, int Wo,
+                        int Ho) {
+  v.resize((size_t)Wo * Ho);
+  for (int y = 0; y < Ho; ++y)
+    for (int x = 0; x < Wo; ++x) std::memcpy(&v[(size_t)y * Wo + x], ptr + y * sx + x * sy, sizeof(T));
+}
+template <typename T>
+static void host_scatter(const std::vector<T>& v, char* ptr, long long sx, long long sy, int Wo,
+                         int Ho) {
+  for (int y = 0; y < Ho; ++y)
+    for (int x = 0; x < Wo; ++x) std::memcpy(ptr + y * sx + x * sy, &v[(size_t)y * Wo + x], sizeof(T));
+}
+
+template <typename T>
+static void run2d(bool fwd, void* ptr, int sx, int sy, int Wo, int Ho, int Wi, int Hi, int* jp,
+                  int d1, int zp, Wavelet w, hipStream_t st, bool sync) {
+  if (!ptr) fail(DWT_GPU_EINVAL, "null image pointer");
+  if (Wo < 0 || Ho < 0 || Wi < 0 || Hi < 0) fail(DWT_GPU_EINVAL, "negative frame size");
+  void* dptr = device_address(ptr);
+  if (dptr) {
+    if (fwd)
+      fwd2_device<T>((char*)dptr, sx, sy, Wo, Ho, Wi, Hi, jp, d1, zp, w, st);
+    else
+      inv2_device<T>((char*)dptr, sx, sy, Wo, Ho, Wi, Hi, *jp, d1, zp, w, st);
+    if (sync) HIPCHK(hipStreamSynchronize(st));
+    return;
+  }
+  // pageable host memory: stage the outer frame as a packed device image
+  if (Wo == 0 || Ho == 0) {
+    if (fwd) {
+      const int jl = level_limit(Wo, Ho, d1);
+      if (*jp < 0 || *jp > jl) *jp = jl;
+    }
+    return;
+  }
+  Scratch& sc = scratch_for(st);
+  const size_t pitch = (size_t)Wo * sizeof(T);
+  char* dev = (char*)sc.get(4, pitch * (size_t)Ho);
+  const bool rows = sy == (int)sizeof(T) && sx > 0 && (size_t)sx >= pitch;
+  std::vector<T> host;
+  if (rows) {
+    HIPCHK(hipMemcpy2DAsync(dev, pitch, ptr, sx, pitch, Ho, hipMemcpyHostToDevice, st));
+  } else {
+    host_gather<T>(host, (const char*)ptr, sx, sy, Wo, Ho);
+    HIPCHK(hipMemcpyAsync(dev, host.data(), pitch * Ho, hipMemcpyHostToDevice, st));
+  }
+  if (fwd)
+    fwd2_device<T>(dev, (long long)pitch, sizeof(T), Wo, Ho, Wi, Hi, jp, d1, zp, w, st);
+  else
+    inv2_device<T>(dev, (long long)pitch, sizeof(T), Wo, Ho, Wi, Hi, *jp, d1, zp, w, st);
+  if (rows) {
+    HIPCHK(hipMemcpy2DAsync(ptr, sx, dev, pitch, pitch, Ho, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  } else {
+    HIPCHK(hipMemcpyAsync(host.data(), dev, pitch * Ho, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    host_scatter<T>(host, (char*)ptr, sx, sy, Wo, Ho);
+  }
+}
+
+// One level of one strided line (src/libdwt.c:5512-5603, 5779-5870).
+// Forward: a = src, b = dst_l, c = dst_h.  Inverse: a = src_l, b = src_h, c = dst.
+template <typename T>
+static void run_line(bool fwd, const void* a, const void* b, void* c, int N, int stride, Wavelet w) {
+  if (N < 0) fail(DWT_GPU_EINVAL, "negative line length");
+  if (stride == 0) fail(DWT_GPU_EINVAL, "zero stride");
+  if (N == 0) return;
+  if (!a || !b || !c) fail(DWT_GPU_EINVAL, "null line pointer");
+  init_once();
+  hipStream_t st = hipStreamPerThread;
+  const Coeffs<T> cf = make_coeffs<T>(w, fwd);
+  Scratch& sc = scratch_for(st);
+  const int lim = line_lds_limit_bytes() / (int)sizeof(T);
+  void* da = device_address(a);
+  void* db = device_address(b);
+  void* dc = device_address(c);
+  if (da && db && dc) {
+    T* gt = N > lim ? (T*)sc.get(3, sizeof(T) * (size_t)N) : nullptr;
+    LAUNCH(launch_line_pass<T>(fwd, (const char*)da, (const char*)db, (char*)dc, 0, stride, 1, N, cf, gt, 1, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return;
+  }
+  if (da || db || dc) fail(DWT_GPU_EMEMORY, "line pointers mix host and device memory");
+  // host line: gather every input sample first (the reference gathers the
+  // whole line into tmp before any store, so any aliasing is well defined),
+  // transform packed on the device, then scatter the outputs
+  const int nl = (N + 1) / 2, nh = N / 2;
+  std::vector<T> in((size_t)N), out((size_t)N);
+  if (fwd) {
+    for (int i = 0; i < N; ++i) std::memcpy(&in[i], (const char*)a + (long long)i * stride, sizeof(T));
+  } else {
+    for (int i = 0; i < nl; ++i) std::memcpy(&in[i], (const char*)a + (long long)i * stride, sizeof(T));
+    for (int i = 0; i < nh; ++i) std::memcpy(&in[nl + i], (const char*)b + (long long)i * stride, sizeof(T));
+  }
+  char* dev = (char*)sc.get(3, sizeof(T) * (size_t)N * 2 + sizeof(T) * (size_t)(N > lim ? N : 0));
+  T* din = (T*)dev;
+  T* dout = din + N;
+  T* gt = N > lim ? dout + N : nullptr;
+  HIPCHK(hipMemcpyAsync(din, in.data(), sizeof(T) * N, hipMemcpyHostToDevice, st));
+  const long long es = sizeof(T);
+  if (fwd)
+    LAUNCH(launch_line_pass<T>(true, (const char*)din, (const char*)dout, (char*)(dout + nl), 0, es, 1, N, cf, gt, 1, st));
+  else
+    LAUNCH(launch_line_pass<T>(false, (const char*)din, (const char*)(din + nl), (char*)dout, 0, es, 1, N, cf, gt, 1, st));
+  HIPCHK(hipMemcpyAsync(out.data(), dout, sizeof(T) * N, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (fwd) {
+    for (int k = 0; k < nl; ++k) std::memcpy((char*)b + (long long)k * stride, &out[k], sizeof(T));
+    for (int k = 0; k < nh; ++k) std::memcpy((char*)c + (long long)k * stride, &out[nl + k], sizeof(T));
+  } else {
+    for (int i = 0; i < N; ++i) std::memcpy((char*)c + (long long)i * stride, &out[i], sizeof(T));
+  }
+}
+
+// Zero [k0, k1) of a strided line (src/libdwt.c:5925-5942, 5988-6004).
+template <typename T>
+static void run_zero(void* p, int k0, int k1, int stride) {
+  if (k1 <= k0) return;
+  if (!p) fail(DWT_GPU_EINVAL, "null pointer");
+  if (stride == 0) fail(DWT_GPU_EINVAL, "zero stride");
+  void* dp = device_address(p);
+  if (!dp) {
+    const T z = 0;
+    for (int k = k0; k < k1; ++k) std::memcpy((char*)p + (long long)k * stride, &z, sizeof(T));
+    return;
+  }
+  hipStream_t st = hipStreamPerThread;
+  const int zx[4] = {k0, k1, 0, 0};
+  const int zy[4] = {0, 0, 0, 0};
+  LAUNCH(launch_zero_pad<T>((char*)dp, 0, stride, k1, 1, zx, zy, st));
+  HIPCHK(hipStreamSynchronize(st));
+}
+
+[[noreturn]] static void abort_with(const char* fn) {
+  std::fprintf(stderr, "libdwt (MI355X): %s: %s\n", fn, g_err.c_str());
+  std::fflush(stderr);
+  std::abort();
+}
+
+}  // namespace dwt
+
+using namespace dwt;
+
+#define DROPIN(fn, body)                 \
+  do {                                   \
+    try {                                \
+      body;                              \
+    } catch (const Failure&) {           \
+      abort_with(fn);                    \
+    }                                    \
+  } while (0)
+
+extern "C" {
+
+// ---- 1D ---------------------------------------------------------------------------
+#define LINE_FNS(W, WENUM, T, SUF)                                                              \
+  void dwt_##W##_f_ex_stride_##SUF(const T* src, T* dst_l, T* dst_h, T* tmp, int N, int stride) { \
+    (void)tmp;                                                                                  \
+    DROPIN(__func__, run_line<T>(true, src, dst_l, dst_h, N, stride, WENUM));                   \
+  }                                                                                             \
+  void dwt_##W##_i_ex_stride_##SUF(const T* src_l, const T* src_h, T* dst, T* tmp, int N,         \
+                                   int stride) {                                                \
+    (void)tmp;                                                                                  \
+    DROPIN(__func__, run_line<T>(false, src_l, src_h, dst, N, stride, WENUM));                  \
+  }                                                                                             \
+  void dwt_##W##_f_ex_##SUF(const T* src, T* dst_l, T* dst_h, T* tmp, int N) {                    \
+    dwt_##W##_f_ex_stride_##SUF(src, dst_l, dst_h, tmp, N, (int)sizeof(T));                     \
+  }                                                                                             \
+  void dwt_##W##_i_ex_##SUF(const T* src_l, const T* src_h, T* dst, T* tmp, int N) {              \
+    dwt_##W##_i_ex_stride_##SUF(src_l, src_h, dst, tmp, N, (int)sizeof(T));                     \
+  }                                                                                             \
+  void dwt_##W##_f_##SUF(const T* src, T* dst, T* tmp, int N) {                                 \
+    dwt_##W##_f_ex_##SUF(src, dst, dst + (N + 1) / 2, tmp, N);                                  \
+  }                                                                                             \
+  void dwt_##W##_i_##SUF(const T* src, T* dst, T* tmp, int N) {                                 \
+    dwt_##W##_i_ex_##SUF(src, src + (N + 1) / 2, dst, tmp, N);                                  \
+  }
+
+LINE_FNS(cdf97, CDF97, float, s)
+LINE_FNS(cdf53, CDF53, float, s)
+LINE_FNS(cdf97, CDF97, double, d)
+LINE_FNS(cdf53, CDF53, double, d)
+
+// ---- zero padding ------------------------------------------------------------------
+#define ZERO_FNS(T, SUF)                                                                          \
+  void dwt_zero_padding_f_stride_##SUF(T* dst_l, T* dst_h, int N, int N_dst_L, int N_dst_H,       \
+                                       int stride) {                                              \
+    DROPIN(__func__, {                                                                            \
+      if (N < 0 || N_dst_L < 0 || N_dst_H < 0) fail(DWT_GPU_EINVAL, "negative size");             \
+      run_zero<T>(dst_l, (N + 1) / 2, N_dst_L, stride);                                           \
+      run_zero<T>(dst_h, N / 2, N_dst_H, stride);                                                 \
+    });                                                                                           \
+  }                                                                                               \
+  void dwt_zero_padding_f_##SUF(T* dst_l, T* dst_h, int N, int N_dst_L, int N_dst_H) {            \
+    dwt_zero_padding_f_stride_##SUF(dst_l, dst_h, N, N_dst_L, N_dst_H, (int)sizeof(T));           \
+  }                                                                                               \
+  void dwt_zero_padding_i_stride_##SUF(T* dst_l, int N, int N_dst, int stride) {                  \
+    DROPIN(__func__, {                                                                            \
+      if (N < 0 || N_dst < 0) fail(DWT_GPU_EINVAL, "negative size");                              \
+      run_zero<T>(dst_l, N, N_dst, stride);                                                       \
+    });                                                                                           \
+  }                                                                                               \
+  void dwt_zero_padding_i_##SUF(T* dst_l, int N, int N_dst) {                                     \
+    dwt_zero_padding_i_stride_##SUF(dst_l, N, N_dst, (int)sizeof(T));                             \
+  }
+
+ZERO_FNS(float, s)
+ZERO_FNS(double, d)
+
+// ---- 2D ------------------------------------------------------------------------------
+#define TWO_D_FNS(W, WENUM, T, SUF)                                                             \
+  void dwt_##W##_2f_##SUF(void* ptr, int stride_x, int stride_y, int size_o_big_x,              \
+                          int size_o_big_y, int size_i_big_x, int size_i_big_y, int* j_max_ptr, \
+                          int decompose_one, int zero_padding) {                                \
+    DROPIN(__func__, {                                                                          \
+      if (!j_max_ptr) fail(DWT_GPU_EINVAL, "null j_max_ptr");                                   \
+      run2d<T>(true, ptr, stride_x, stride_y, size_o_big_x, size_o_big_y, size_i_big_x,         \
+               size_i_big_y, j_max_ptr, decompose_one, zero_padding, WENUM, hipStreamPerThread, \
+               true);                                                                           \
+    });                                                                                         \
+  }                                                                                             \
+  void dwt_##W##_2i_##SUF(void* ptr, int stride_x, int stride_y, int size_o_big_x,              \
+                          int size_o_big_y, int size_i_big_x, int size_i_big_y, int j_max,      \
+                          int decompose_one, int zero_padding) {                                \
+    DROPIN(__func__, {                                                                          \
+      int j = j_max;                                                                            \
+      run2d<T>(false, ptr, stride_x, stride_y, size_o_big_x, size_o_big_y, size_i_big_x,        \
+               size_i_big_y, &j, decompose_one, zero_padding, WENUM, hipStreamPerThread, true); \
+    });                                                                                         \
+  }
+
+TWO_D_FNS(cdf97, CDF97, float, s)
+TWO_D_FNS(cdf53, CDF53, float, s)
+TWO_D_FNS(cdf97, CDF97, double, d)
+TWO_D_FNS(cdf53, CDF53, double, d)
+
+// ---- extensions ------------------------------------------------------------------------
+int dwt_gpu_init(void) {
+  try {
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (n <= 0) fail(DWT_GPU_EHIP, "no HIP device");
+    init_once();
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+const char* dwt_gpu_last_error(void) { return g_err.c_str(); }
+
+const char* dwt_gpu_version(void) { return "libdwt-mi355x 0.1 (gfx950)"; }
+
+int dwt_gpu_2d_async(int wavelet, int direction, int precision, void* ptr, int stride_x,
+                     int stride_y, int size_o_big_x, int size_o_big_y, int size_i_big_x,
+                     int size_i_big_y, int* j, int decompose_one, int zero_padding, void* stream) {
+  try {
+    g_err.clear();
+    if (wavelet != DWT_GPU_CDF97 && wavelet != DWT_GPU_CDF53) fail(DWT_GPU_EINVAL, "bad wavelet");
+    if (direction != DWT_GPU_FORWARD && direction != DWT_GPU_INVERSE) fail(DWT_GPU_EINVAL, "bad direction");
+    if (precision != DWT_GPU_F32 && precision != DWT_GPU_F64) fail(DWT_GPU_EINVAL, "bad precision");
+    if (!j) fail(DWT_GPU_EINVAL, "null j");
+    if (!ptr) fail(DWT_GPU_EINVAL, "null image pointer");
+    void* dptr = device_address(ptr);
+    if (!dptr) fail(DWT_GPU_EMEMORY, "image pointer is not device-accessible memory");
+    const Wavelet w = wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53;
+    const bool fwd = direction == DWT_GPU_FORWARD;
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == DWT_GPU_F32)
+      run2d<float>(fwd, dptr, stride_x, stride_y, size_o_big_x, size_o_big_y, size_i_big_x,
+                   size_i_big_y, j, decompose_one, zero_padding, w, st, false);
+    else
+      run2d<double>(fwd, dptr, stride_x, stride_y, size_o_big_x, size_o_big_y, size_i_big_x,
+                    size_i_big_y, j, decompose_one, zero_padding, w, st, false);
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_profile_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_on = on != 0;
+  return DWT_GPU_OK;
+}
+
+int dwt_gpu_profile_read(int kind, int level, long long* launches, double* total_ms) {
+  try {
+    std::vector<ProfRec> recs;
+    {
+      std::lock_guard<std::mutex> lk(g_prof_mu);
+      recs = *g_prof;
+    }
+    long long n = 0;
+    double ms = 0;
+    for (const auto& r : recs) {
+      if (r.kind != kind || (level >= 0 && r.level != level)) continue;
+      HIPCHK(hipEventSynchronize(r.b));
+      float t = 0;
+      HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+      ms += t;
+      ++n;
+    }
+    if (launches) *launches = n;
+    if (total_ms) *total_ms = ms;
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_profile_reset(void) {
+  std::vector<ProfRec> recs;
+  {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    recs.swap(*g_prof);
+  }
+  for (auto& r : recs) {
+    hipEventSynchronize(r.b);
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  return DWT_GPU_OK;
+}
+
+int dwt_gpu_tile_set(int forward, int height_class, int seg_rows, int prefetch_pairs) {
+  if (tile_set(forward, height_class, seg_rows, prefetch_pairs) != 0) {
+    g_err = "dwt_gpu_tile_set: bad class or shape";
+    return DWT_GPU_EINVAL;
+  }
+  return DWT_GPU_OK;
+}
+
+int dwt_gpu_tile_get(int forward, int width, int height, int wavelet, int* seg_rows,
+                     int* prefetch_pairs) {
+  if (!seg_rows || !prefetch_pairs || width < 1 || height < 1) return DWT_GPU_EINVAL;
+  return tile_get(forward, height, width, wavelet == DWT_GPU_CDF53 ? 2 : 4, seg_rows, prefetch_pairs);
+}
+
+long long dwt_gpu_scratch_bytes(void* stream) {
+  try {
+    return scratch_for((hipStream_t)stream).bytes();
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_release_scratch(void* stream) {
+  try {
+    Scratch& s = scratch_for((hipStream_t)stream);
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    s.release();
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+}  // extern "C"

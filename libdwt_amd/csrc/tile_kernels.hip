@@ -1,0 +1,465 @@
+// Packed fast path: one 2D lifting level per launch, one HBM read of the
+// level input and one write of its four subbands.
+//
+// Work decomposition.  A wavefront owns a vertical strip of 128 window
+// columns: lane l holds the sample pair (2l, 2l+1) of the window, which
+// starts HALO samples left of the strip's core, so the core is the
+// 128 - 2*HALO columns whose lifting is complete after NS steps (HALO = NS:
+// 4 for CDF 9/7, 2 for CDF 5/3).  The wave walks down a segment of S rows
+// (plus HALO rows above and below), two rows at a time:
+//   * horizontal lifting of each row runs in registers; the neighbour sample
+//     across a lane boundary comes from the adjacent lane (wavefront
+//     shuffle), so no LDS is used;
+//   * vertical lifting is a per-lane sliding window: each new row pair
+//     advances every lifting step by one position and completes one output
+//     pair (L, H) of every column, so the column pass costs no extra memory
+//     traffic either.
+// Line ends use whole-sample symmetric extension: rows and columns outside
+// the image are read from their mirror images, which reproduces the
+// reference's end-of-line update  x + (2c) * n  bit for bit.
+//
+// Four waves of a workgroup take four adjacent strips of the same segment,
+// and the block index is remapped so that blocks resident on one XCD walk
+// down neighbouring segments (their halo rows then hit that XCD's L2).
+#include "dwt_common.h"
+#include "kernels.h"
+
+namespace dwt {
+
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+
+// Hardware assigns block b to XCD (b mod 8); give every XCD a contiguous run
+// of logical tiles.
+__device__ inline int xcd_remap(int b, int nb) {
+  const int xcd = b & 7, idx = b >> 3;
+  const int per = nb >> 3, rem = nb & 7;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+// Value held by lane+1 / lane-1 (the edge lanes get their own value; those
+// lanes are halo lanes whose results are never stored).
+template <typename T>
+__device__ inline T from_next_lane(T v) { return __shfl_down(v, 1, kWave); }
+template <typename T>
+__device__ inline T from_prev_lane(T v) { return __shfl_up(v, 1, kWave); }
+
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+
+// Forward lifting of one row: lane holds (even, odd) samples; odd samples are
+// updated first.  Result: (L, H) coefficient of the lane's pair.
+template <typename T, int NS>
+__device__ inline void row_fwd(T& e, T& o, const Coeffs<T>& cf) {
+#pragma unroll
+  for (int s = 0; s < NS; s += 2) {
+    {
+      T t = e + from_next_lane(e);
+      t = cf.c[s] * t;
+      o = o + t;
+    }
+    {
+      T t = from_prev_lane(o) + o;
+      t = cf.c[s + 1] * t;
+      e = e + t;
+    }
+  }
+  e = e * cf.se;
+  o = o * cf.so;
+}
+
+// Inverse lifting of one interleaved row: scale, then even samples first.
+template <typename T, int NS>
+__device__ inline void row_inv(T& e, T& o, const Coeffs<T>& cf) {
+  e = e * cf.se;
+  o = o * cf.so;
+#pragma unroll
+  for (int s = 0; s < NS; s += 2) {
+    {
+      T t = from_prev_lane(o) + o;
+      t = cf.c[s] * t;
+      e = e + t;
+    }
+    {
+      T t = e + from_next_lane(e);
+      t = cf.c[s + 1] * t;
+      o = o + t;
+    }
+  }
+}
+
+// Vertical forward lifting of one column as a sliding window.  push() takes
+// the samples of rows 2p and 2p+1 and returns the finished pair p - NS/2.
+template <typename T, int NS>
+struct ColFwd {
+  T e0 = 0, o0 = 0, o1 = 0, e2 = 0, o3 = 0;
+  __device__ inline void push(T ze, T zo, const Coeffs<T>& cf, T& L, T& H) {
+    T t1 = e0 + ze;
+    t1 = cf.c[0] * t1;
+    t1 = o0 + t1;          // step 1 at row 2p-1
+    T t2 = o1 + t1;
+    t2 = cf.c[1] * t2;
+    t2 = e0 + t2;          // step 2 at row 2p-2
+    if (NS == 4) {
+      T t3 = e2 + t2;
+      t3 = cf.c[2] * t3;
+      t3 = o1 + t3;        // step 3 at row 2p-3
+      T t4 = o3 + t3;
+      t4 = cf.c[3] * t4;
+      t4 = e2 + t4;        // step 4 at row 2p-4
+      L = t4 * cf.se;
+      H = t3 * cf.so;
+      e2 = t2;
+      o3 = t3;
+    } else {
+      L = t2 * cf.se;
+      H = t1 * cf.so;
+    }
+    e0 = ze;
+    o0 = zo;
+    o1 = t1;
+  }
+};
+
+// Vertical inverse lifting as a sliding window.  push() takes the (unscaled)
+// coefficients of interleaved rows 2p (L) and 2p+1 (H) and returns two
+// finished output rows: (2p-3, 2p-2) for NS = 4, (2p-2, 2p-1) for NS = 2,
+// as (first, second).
+template <typename T, int NS>
+struct ColInv {
+  T zp = 0, e1 = 0, o2 = 0, e3 = 0;
+  __device__ inline void push(T ze, T zo, const Coeffs<T>& cf, T& first, T& second) {
+    ze = ze * cf.se;
+    zo = zo * cf.so;
+    T a1 = zp + zo;
+    a1 = cf.c[0] * a1;
+    a1 = ze + a1;          // even step at row 2p
+    T a2 = e1 + a1;
+    a2 = cf.c[1] * a2;
+    a2 = zp + a2;          // odd step at row 2p-1
+    if (NS == 4) {
+      T a3 = o2 + a2;
+      a3 = cf.c[2] * a3;
+      a3 = e1 + a3;        // even step at row 2p-2
+      T a4 = e3 + a3;
+      a4 = cf.c[3] * a4;
+      a4 = o2 + a4;        // odd step at row 2p-3
+      first = a4;
+      second = a3;
+      o2 = a2;
+      e3 = a3;
+    } else {
+      first = e1;
+      second = a2;
+    }
+    zp = zo;
+    e1 = a1;
+  }
+};
+
+template <typename T, int NS, bool VEC, int U>
+__global__ void __launch_bounds__(256)
+fwd_level_kernel(const T* __restrict__ src, long long sp, int W, int H, T* __restrict__ ll,
+                 long long llp, T* __restrict__ det, long long dp, int S, int nstrips, int nseg,
+                 Coeffs<T> cf) {
+  constexpr int HALO = NS;
+  constexpr int CORE = 2 * kWave - 2 * HALO;
+  constexpr int LAG = NS / 2;
+  using V2 = typename Vec2<T>::type;
+
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int g = b / nseg, s = b - g * nseg;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
+  if (strip >= nstrips) return;
+
+  const int x0 = strip * CORE;
+  const int Xe = x0 - HALO + 2 * lane;
+  const int xe = mirror(Xe, W), xo = mirror(Xe + 1, W);
+  const int Wd = (W + 1) >> 1, Wh = W >> 1, Hd = (H + 1) >> 1, Hh = H >> 1;
+  const int k = Xe >> 1;
+  const bool core = lane >= HALO / 2 && lane < kWave - HALO / 2;
+  const bool st_a = core && k < Wd;
+  const bool st_b = core && k < Wh;
+  const bool vec = VEC && (x0 - HALO >= 0) && (x0 - HALO + 2 * kWave <= W);
+
+  const int m0 = (s * S) >> 1;
+  const int m1 = min((s * S + S) >> 1, Hd);
+  const int pbeg = m0 - LAG, pend = m1 + LAG - 1;
+
+  ColFwd<T, NS> ca, cb;
+  T ce[2 * U], co[2 * U];
+
+  auto load_row = [&](int Y, T& e, T& o) {
+    const T* rp = src + (long long)mirror(Y, H) * sp;
+    if (vec) {
+      const V2 v = *reinterpret_cast<const V2*>(rp + Xe);
+      e = v.x;
+      o = v.y;
+    } else {
+      e = rp[xe];
+      o = rp[xo];
+    }
+  };
+
+#pragma unroll
+  for (int u = 0; u < 2 * U; ++u) load_row(2 * pbeg + u, ce[u], co[u]);
+
+  for (int pb = pbeg; pb <= pend; pb += U) {
+    T ne[2 * U], no[2 * U];
+#pragma unroll
+    for (int u = 0; u < 2 * U; ++u) load_row(2 * (pb + U) + u, ne[u], no[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      T e0 = ce[2 * u], o0 = co[2 * u], e1 = ce[2 * u + 1], o1 = co[2 * u + 1];
+      row_fwd<T, NS>(e0, o0, cf);
+      row_fwd<T, NS>(e1, o1, cf);
+      T La, Ha, Lb, Hb;
+      ca.push(e0, e1, cf, La, Ha);
+      cb.push(o0, o1, cf, Lb, Hb);
+      const int m = pb + u - LAG;
+      if (m >= m0 && m < m1) {
+        if (st_a) {
+          ll[(long long)m * llp + k] = La;
+          if (m < Hh) det[(long long)(Hd + m) * dp + k] = Ha;
+        }
+        if (st_b) {
+          det[(long long)m * dp + Wd + k] = Lb;
+          if (m < Hh) det[(long long)(Hd + m) * dp + Wd + k] = Hb;
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2 * U; ++u) {
+      ce[u] = ne[u];
+      co[u] = no[u];
+    }
+  }
+}
+
+template <typename T, int NS, bool VEC, int U>
+__global__ void __launch_bounds__(256)
+inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ det, long long dp,
+                 T* __restrict__ dst, long long dstp, int W, int H, int S, int nstrips, int nseg,
+                 Coeffs<T> cf) {
+  constexpr int HALO = NS;
+  constexpr int CORE = 2 * kWave - 2 * HALO;
+  constexpr int LAG = NS / 2;
+  using V2 = typename Vec2<T>::type;
+
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int g = b / nseg, s = b - g * nseg;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
+  if (strip >= nstrips) return;
+
+  const int x0 = strip * CORE;
+  const int Xe = x0 - HALO + 2 * lane;
+  const int ke = mirror(Xe, W) >> 1, ko = mirror(Xe + 1, W) >> 1;
+  const int Wc = (W + 1) >> 1, Hc = (H + 1) >> 1;
+  const bool core = lane >= HALO / 2 && lane < kWave - HALO / 2;
+  const bool st_e = core && Xe < W;
+  const bool st_o = core && Xe + 1 < W;
+  const bool vec = VEC && st_o;
+
+  const int y0 = s * S;
+  const int y1 = min(y0 + S, H);
+  const int pbeg = (y0 >> 1) - LAG, pend = (y1 >> 1) + LAG + 1;
+
+  ColInv<T, NS> cle, clo;
+  T ce[2 * U], co[2 * U];
+
+  auto load_row = [&](int Y, T& e, T& o) {
+    const int Yr = mirror(Y, H);
+    const int m = Yr >> 1;
+    if (Yr & 1) {
+      const T* r = det + (long long)(Hc + m) * dp;
+      e = r[ke];
+      o = r[Wc + ko];
+    } else {
+      e = ll[(long long)m * llp + ke];
+      o = det[(long long)m * dp + Wc + ko];
+    }
+  };
+  auto store_row = [&](int r, T e, T o) {
+    if (r < y0 || r >= y1) return;
+    T* rp = dst + (long long)r * dstp + Xe;
+    if (vec) {
+      V2 v;
+      v.x = e;
+      v.y = o;
+      *reinterpret_cast<V2*>(rp) = v;
+    } else {
+      if (st_e) rp[0] = e;
+      if (st_o) rp[1] = o;
+    }
+  };
+
+#pragma unroll
+  for (int u = 0; u < 2 * U; ++u) load_row(2 * pbeg + u, ce[u], co[u]);
+
+  for (int pb = pbeg; pb <= pend; pb += U) {
+    T ne[2 * U], no[2 * U];
+#pragma unroll
+    for (int u = 0; u < 2 * U; ++u) load_row(2 * (pb + U) + u, ne[u], no[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      T e0 = ce[2 * u], o0 = co[2 * u], e1 = ce[2 * u + 1], o1 = co[2 * u + 1];
+      row_inv<T, NS>(e0, o0, cf);
+      row_inv<T, NS>(e1, o1, cf);
+      T fe, se_, fo, so_;
+      cle.push(e0, e1, cf, fe, se_);
+      clo.push(o0, o1, cf, fo, so_);
+      const int p = pb + u;
+      const int r0 = NS == 4 ? 2 * p - 3 : 2 * p - 2;
+      if (core) {
+        store_row(r0, fe, fo);
+        store_row(r0 + 1, se_, so_);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2 * U; ++u) {
+      ce[u] = ne[u];
+      co[u] = no[u];
+    }
+  }
+}
+
+template <typename T>
+__global__ void copy_rect_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst,
+                                 long long dp, int w, int h) {
+  const long long n = (long long)w * h;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int y = (int)(t / w), x = (int)(t - (long long)y * w);
+    dst[y * dp + x] = src[y * sp + x];
+  }
+}
+
+// ---- host launchers ---------------------------------------------------------
+
+// Tile shape of one launch: S rows per segment (tall segments amortise the
+// 2*HALO halo rows, short ones give a level more wavefronts) and U row pairs
+// prefetched per wavefront (U = 8 keeps more bytes in flight and lets a short
+// segment issue all its loads at once).  The table is indexed by the level's
+// height class ceil(log2(H)); entries are set by dwt_gpu_tile_set (the
+// autotuner) and default to the heuristic below.
+struct TileShape {
+  int S, U;
+};
+static TileShape g_tiles[2][32];  // [direction][height class]; S == 0: heuristic
+static int height_class(int H) {
+  int c = 0;
+  while ((1 << c) < H && c < 31) ++c;
+  return c;
+}
+
+static TileShape pick_tile(int H, int nstrips, bool fwd) {
+  const TileShape t = g_tiles[fwd ? 0 : 1][height_class(H)];
+  if (t.S > 0) return t;
+  int S = 64;
+  while (S > 8 && (long long)nstrips * ((H + S - 1) / S) < 2048) S >>= 1;
+  return TileShape{S, S <= 16 ? 8 : 4};
+}
+
+int tile_set(int fwd, int hclass, int S, int U) {
+  if (hclass < 0 || hclass >= 32) return -1;
+  if (S != 0 && (S < 2 || (S & 1) || S > 4096)) return -1;
+  if (S != 0 && U != 4 && U != 8) return -1;
+  g_tiles[fwd ? 0 : 1][hclass] = TileShape{S, U};
+  return 0;
+}
+
+int tile_get(int fwd, int H, int W, int nsteps, int* S, int* U) {
+  const int core = 2 * kWave - 2 * nsteps;
+  const TileShape t = pick_tile(H, (W + core - 1) / core, fwd != 0);
+  *S = t.S;
+  *U = t.U;
+  return 0;
+}
+
+template <typename T>
+static bool aligned_pair(const void* p, long long pitch) {
+  return (reinterpret_cast<uintptr_t>(p) % (2 * sizeof(T))) == 0 && (pitch % 2) == 0;
+}
+
+#define DWT_DISPATCH_FWD(NS, VEC, U)                                                        \
+  hipLaunchKernelGGL((fwd_level_kernel<T, NS, VEC, U>), grid, block, 0, st, src, sp, W, H, ll, \
+                     llp, det, dp, t.S, nstrips, nseg, cf)
+#define DWT_DISPATCH_INV(NS, VEC, U)                                                           \
+  hipLaunchKernelGGL((inv_level_kernel<T, NS, VEC, U>), grid, block, 0, st, ll, llp, det, dp, dst, \
+                     dstp, W, H, t.S, nstrips, nseg, cf)
+
+template <typename T>
+void launch_fwd_level(const T* src, long long sp, int W, int H, T* ll, long long llp, T* det,
+                      long long dp, const Coeffs<T>& cf, hipStream_t st) {
+  const int core = 2 * kWave - 2 * cf.nsteps;
+  const int nstrips = (W + core - 1) / core;
+  const TileShape t = pick_tile(H, nstrips, true);
+  const int nseg = (H + t.S - 1) / t.S;
+  const int ngroups = (nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
+  const dim3 grid(ngroups * nseg), block(kWave * kWavesPerBlock);
+  const bool vec = aligned_pair<T>(src, sp);
+  const int sel = (cf.nsteps == 4 ? 4 : 0) + (vec ? 2 : 0) + (t.U == 8 ? 1 : 0);
+  switch (sel) {
+    case 0: DWT_DISPATCH_FWD(2, false, 4); break;
+    case 1: DWT_DISPATCH_FWD(2, false, 8); break;
+    case 2: DWT_DISPATCH_FWD(2, true, 4); break;
+    case 3: DWT_DISPATCH_FWD(2, true, 8); break;
+    case 4: DWT_DISPATCH_FWD(4, false, 4); break;
+    case 5: DWT_DISPATCH_FWD(4, false, 8); break;
+    case 6: DWT_DISPATCH_FWD(4, true, 4); break;
+    default: DWT_DISPATCH_FWD(4, true, 8); break;
+  }
+}
+
+template <typename T>
+void launch_inv_level(const T* ll, long long llp, const T* det, long long dp, T* dst,
+                      long long dstp, int W, int H, const Coeffs<T>& cf, hipStream_t st) {
+  const int core = 2 * kWave - 2 * cf.nsteps;
+  const int nstrips = (W + core - 1) / core;
+  const TileShape t = pick_tile(H, nstrips, false);
+  const int nseg = (H + t.S - 1) / t.S;
+  const int ngroups = (nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
+  const dim3 grid(ngroups * nseg), block(kWave * kWavesPerBlock);
+  const bool vec = aligned_pair<T>(dst, dstp);
+  const int sel = (cf.nsteps == 4 ? 4 : 0) + (vec ? 2 : 0) + (t.U == 8 ? 1 : 0);
+  switch (sel) {
+    case 0: DWT_DISPATCH_INV(2, false, 4); break;
+    case 1: DWT_DISPATCH_INV(2, false, 8); break;
+    case 2: DWT_DISPATCH_INV(2, true, 4); break;
+    case 3: DWT_DISPATCH_INV(2, true, 8); break;
+    case 4: DWT_DISPATCH_INV(4, false, 4); break;
+    case 5: DWT_DISPATCH_INV(4, false, 8); break;
+    case 6: DWT_DISPATCH_INV(4, true, 4); break;
+    default: DWT_DISPATCH_INV(4, true, 8); break;
+  }
+}
+
+template <typename T>
+void launch_copy_rect(const T* src, long long sp, T* dst, long long dp, int w, int h,
+                      hipStream_t st) {
+  const long long n = (long long)w * h;
+  if (n <= 0) return;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL((copy_rect_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, src, sp,
+                     dst, dp, w, h);
+}
+
+template void launch_fwd_level<float>(const float*, long long, int, int, float*, long long,
+                                      float*, long long, const Coeffs<float>&, hipStream_t);
+template void launch_fwd_level<double>(const double*, long long, int, int, double*, long long,
+                                       double*, long long, const Coeffs<double>&, hipStream_t);
+template void launch_inv_level<float>(const float*, long long, const float*, long long, float*,
+                                      long long, int, int, const Coeffs<float>&, hipStream_t);
+template void launch_inv_level<double>(const double*, long long, const double*, long long,
+                                       double*, long long, int, int, const Coeffs<double>&,
+                                       hipStream_t);
+template void launch_copy_rect<float>(const float*, long long, float*, long long, int, int,
+                                      hipStream_t);
+template void launch_copy_rect<double>(const double*, long long, double*, long long, int, int,
+                                       hipStream_t);
+
+}  // namespace dwt

@@ -1,0 +1,144 @@
+"""GPU parity: the HIP library against the reference's golden digests and the
+oracle, through the C ABI.
+
+Bar: bit-exact (NaN-free inputs) to the reference's _s and _d outputs for
+every case; FP32 results within 1e-5 max-abs of the FP64 oracle on the
+benchmark images.
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+import libdwt_amd
+from libdwt_amd import workloads
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+FP32_TOL = 1e-5  # max-abs vs the FP64 oracle on the sinusoid+noise images
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def gold():
+    with open(os.path.join(GOLD, "golden_hashes.json")) as fh:
+        return json.load(fh)
+
+
+@pytest.fixture(scope="module")
+def arrays():
+    return np.load(os.path.join(GOLD, "golden_arrays.npz"))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    libdwt_amd.lib()
+    assert libdwt_amd.lib().dwt_gpu_init() == 0, libdwt_amd.lib().dwt_gpu_last_error()
+    return torch.device("cuda:0")
+
+
+def dropin(mode, timeout=240):
+    """Run the drop-in entry points over the fixture cases in a child process:
+    they abort on error like the reference, and a child keeps that abort (and
+    the library's message, and the case it names) contained and visible."""
+    worker = os.path.join(os.path.dirname(__file__), "gpu_dropin_worker.py")
+    r = subprocess.run([sys.executable, "-u", worker, mode], capture_output=True, text=True,
+                       timeout=timeout)
+    tail = "\n".join(r.stderr.strip().splitlines()[-6:])
+    assert r.returncode == 0, f"worker {mode} exited {r.returncode}:\n{tail}"
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    say(f"  {mode}: {res['cases']} cases, {len(res['mismatches'])} mismatches")
+    return res
+
+
+@pytest.mark.parametrize("mode", ["dense", "sparse", "host", "lines", "slice", "pinned", "example"])
+def test_dropin_entry_points(mode):
+    """Dense frames, sparse frames / zero padding / strided layouts, staged
+    host memory, 1D lines N = 0..40 (strided, aliased), the 1D full-depth
+    slice, pinned host memory and the reference example's 2053-byte pitch,
+    all through the drop-in C ABI, against the reference's digests."""
+    say(f"drop-in sweep: {mode}")
+    res = dropin(mode)
+    assert res["cases"] > 0
+    assert not res["mismatches"], res["mismatches"][:8]
+
+
+@pytest.mark.parametrize("cfg", cases.CONFIG_IMAGES, ids=lambda c: c["name"])
+@pytest.mark.parametrize("wavelet", cases.WAVELETS)
+def test_config_images(gold, cfg, wavelet):
+    from oracle.oracle import oracle
+    say(f"config image {cfg['name']} {wavelet}")
+    img = workloads.sinusoid_image(cfg["W"], cfg["H"], cfg["seed"])
+    rec = gold["config"][cfg["name"]]
+    assert cases.digest(img) == rec["input"]
+    t = torch.from_numpy(img).cuda()
+    j = libdwt_amd.fwd2(t, wavelet, cfg["J"])
+    torch.cuda.synchronize()
+    coef = t.cpu().numpy()
+    assert j == rec[wavelet]["j"]
+    assert cases.digest(coef) == rec[wavelet]["fwd"], "forward differs from the reference _s"
+    # FP32 vs FP64 oracle
+    o = oracle()
+    cd, _ = o.fwd2(img.astype(np.float64), wavelet, cfg["J"])
+    err = float(np.abs(coef - cd).max())
+    say(f"  max-abs vs FP64 oracle: {err:.3e}")
+    assert err <= FP32_TOL
+    libdwt_amd.inv2(t, wavelet, j)
+    torch.cuda.synchronize()
+    rec_img = t.cpu().numpy()
+    assert cases.digest(rec_img) == rec[wavelet]["inv"], "inverse differs from the reference _s"
+    assert float(np.abs(rec_img - img).max()) <= FP32_TOL
+
+
+def test_fp64_config_image_matches_oracle():
+    from oracle.oracle import oracle
+    say("fp64 512x512 image")
+    img = workloads.sinusoid_image(512, 512, 1).astype(np.float64)
+    o = oracle()
+    for w in cases.WAVELETS:
+        t = torch.from_numpy(img).cuda()
+        j = libdwt_amd.fwd2(t, w, 5)
+        torch.cuda.synchronize()
+        ref, _ = o.fwd2(img, w, 5)
+        np.testing.assert_array_equal(t.cpu().numpy(), ref)
+        libdwt_amd.inv2(t, w, j)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(t.cpu().numpy(), o.inv2(ref, w, j))
+
+
+def test_async_api_on_torch_stream(gold):
+    """Three forward+inverse round trips and a final forward on a side stream,
+    enqueued without synchronisation; equal to the oracle doing the same."""
+    from oracle.oracle import oracle
+    say("async API on a torch stream")
+    cfg = cases.CONFIG_IMAGES[1]
+    img = workloads.sinusoid_image(cfg["W"], cfg["H"], cfg["seed"])
+    o = oracle()
+    want = img
+    for _ in range(3):
+        c, j = o.fwd2(want, "cdf97", cfg["J"])
+        want = o.inv2(c, "cdf97", j)
+    want, _ = o.fwd2(want, "cdf97", cfg["J"])
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        t = torch.from_numpy(img).cuda()
+        for _ in range(3):
+            j = libdwt_amd.transform_2d_async("cdf97", True, t.data_ptr(), 4 * cfg["W"], 4,
+                                              cfg["W"], cfg["H"], j=cfg["J"], stream=s.cuda_stream)
+            libdwt_amd.transform_2d_async("cdf97", False, t.data_ptr(), 4 * cfg["W"], 4,
+                                          cfg["W"], cfg["H"], j=j, stream=s.cuda_stream)
+        libdwt_amd.transform_2d_async("cdf97", True, t.data_ptr(), 4 * cfg["W"], 4, cfg["W"],
+                                      cfg["H"], j=cfg["J"], stream=s.cuda_stream)
+    s.synchronize()
+    np.testing.assert_array_equal(t.cpu().numpy(), want)
