@@ -58,15 +58,28 @@ __device__ void scale_line(T* x, int N, T se, T so) {
 // Forward: read src (N samples), write L (ceil(N/2)) to dl and H (floor(N/2))
 // to dh.  Inverse: read L from a, H from b, write N samples to c.
 // Line `l` of the pass starts at base + l * line_step for each of a, b, c;
-// samples are es bytes apart.  gtmp: per-block global line buffer when the line
-// does not fit the dynamic LDS (nullptr otherwise).
-template <typename T, bool FWD>
+// samples are es bytes apart.
+//
+// The line buffer is either the dynamic LDS (IN_LDS) or, for lines longer
+// than the LDS limit, a per-block slice of the global buffer gtmp.  The two
+// are separate instantiations on purpose: a pointer that may point to either
+// is a generic (flat) pointer, and flat access to LDS breaks when the
+// compiler folds x[i-1] into base (x - 4) plus an instruction offset of 4 --
+// for i = 0 that base lies below the start of the LDS aperture, which the
+// hardware rejects as an aperture violation.  With IN_LDS the buffer is only
+// ever addressed by ds_* instructions.
+template <typename T, bool FWD, bool IN_LDS>
 __global__ void __launch_bounds__(256) line_pass_kernel(const char* a, const char* b, char* c,
                                                         long long line_step, long long es,
                                                         int nlines, int N, Coeffs<T> cf,
                                                         T* gtmp) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* x = gtmp ? gtmp + (size_t)blockIdx.x * (size_t)N : reinterpret_cast<T*>(smem_raw);
+  T* x;
+  if constexpr (IN_LDS) {
+    x = reinterpret_cast<T*>(smem_raw);
+  } else {
+    x = gtmp + (size_t)blockIdx.x * (size_t)N;
+  }
   for (int line = blockIdx.x; line < nlines; line += gridDim.x) {
     const long long lo = (long long)line * line_step;
     const char* pa = a + lo;
@@ -127,13 +140,23 @@ void launch_line_pass(bool fwd, const char* a, const char* b, char* c, long long
                       long long es, int nlines, int N, const Coeffs<T>& cf, T* gtmp,
                       int gtmp_blocks, hipStream_t st) {
   if (nlines <= 0 || N <= 0) return;
-  const size_t lds = gtmp ? 0 : (size_t)N * sizeof(T);
-  const int grid = gtmp ? (nlines < gtmp_blocks ? nlines : gtmp_blocks) : (nlines < 65535 * 8 ? nlines : 65535 * 8);
+  if (gtmp) {
+    const int grid = nlines < gtmp_blocks ? nlines : gtmp_blocks;
+    if (fwd)
+      hipLaunchKernelGGL((line_pass_kernel<T, true, false>), dim3(grid), dim3(256), 0, st, a, b, c,
+                         line_step, es, nlines, N, cf, gtmp);
+    else
+      hipLaunchKernelGGL((line_pass_kernel<T, false, false>), dim3(grid), dim3(256), 0, st, a, b,
+                         c, line_step, es, nlines, N, cf, gtmp);
+    return;
+  }
+  const size_t lds = (size_t)N * sizeof(T);
+  const int grid = nlines < 65535 * 8 ? nlines : 65535 * 8;
   if (fwd)
-    hipLaunchKernelGGL((line_pass_kernel<T, true>), dim3(grid), dim3(256), lds, st, a, b, c,
+    hipLaunchKernelGGL((line_pass_kernel<T, true, true>), dim3(grid), dim3(256), lds, st, a, b, c,
                        line_step, es, nlines, N, cf, gtmp);
   else
-    hipLaunchKernelGGL((line_pass_kernel<T, false>), dim3(grid), dim3(256), lds, st, a, b, c,
+    hipLaunchKernelGGL((line_pass_kernel<T, false, true>), dim3(grid), dim3(256), lds, st, a, b, c,
                        line_step, es, nlines, N, cf, gtmp);
 }
 
@@ -152,10 +175,10 @@ int line_lds_limit_bytes() { return 64 * 1024; }
 
 hipError_t configure_line_kernels() {
   const int bytes = line_lds_limit_bytes();
-  const void* fns[4] = {reinterpret_cast<const void*>(&line_pass_kernel<float, true>),
-                        reinterpret_cast<const void*>(&line_pass_kernel<float, false>),
-                        reinterpret_cast<const void*>(&line_pass_kernel<double, true>),
-                        reinterpret_cast<const void*>(&line_pass_kernel<double, false>)};
+  const void* fns[4] = {reinterpret_cast<const void*>(&line_pass_kernel<float, true, true>),
+                        reinterpret_cast<const void*>(&line_pass_kernel<float, false, true>),
+                        reinterpret_cast<const void*>(&line_pass_kernel<double, true, true>),
+                        reinterpret_cast<const void*>(&line_pass_kernel<double, false, true>)};
   for (const void* f : fns) {
     const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
     if (e != hipSuccess) {
