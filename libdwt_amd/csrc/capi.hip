@@ -210,7 +210,8 @@ static Scratch& scratch_for(hipStream_t st) {
 
 // ---- optional per-launch timing (HIP events on the launch stream) ---------------
 // Kinds: 0 forward level kernel, 1 inverse level kernel, 2 detail copy,
-// 3 generic line pass.  Level is the level index the launch belongs to.
+// 4 forward tail kernel, 5 inverse tail kernel.  Level is the level index the
+// launch belongs to (the first tail level for a tail).
 struct ProfRec {
   int kind, level;
   hipEvent_t a, b;
@@ -246,9 +247,12 @@ struct ProfScope {
 static std::once_flag g_init_once;
 static hipError_t g_init_status = hipSuccess;
 static void init_once() {
-  std::call_once(g_init_once, [] { g_init_status = configure_line_kernels(); });
+  std::call_once(g_init_once, [] {
+    g_init_status = configure_line_kernels();
+    if (g_init_status == hipSuccess) g_init_status = configure_tail_kernels();
+  });
   if (g_init_status != hipSuccess)
-    fail(DWT_GPU_EHIP, "setting the line kernels' LDS limit: %s", hipGetErrorString(g_init_status));
+    fail(DWT_GPU_EHIP, "raising the kernels' LDS limits: %s", hipGetErrorString(g_init_status));
 }
 
 static size_t pad_pitch(size_t w) { return (w + 63) & ~size_t(63); }
@@ -312,7 +316,8 @@ static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
   if (L >= 2) P[1] = (T*)sc.get(1, sizeof(T) * (size_t)Pp[1] * H1);
   if (L >= 3) P[0] = (T*)sc.get(0, sizeof(T) * (size_t)Pp[0] * H2);
   sc.ensure_side();
-  for (int j = 0; j < L; ++j) {
+  const int jt = tail_start(W, H, L, (int)sizeof(T));
+  for (int j = 0; j < jt; ++j) {
     const int Wj = ceil_div_pow2(W, j), Hj = ceil_div_pow2(H, j);
     const T* src = j == 0 ? A : P[j % 2];
     const long long sp = j == 0 ? p : Pp[j % 2];
@@ -341,6 +346,12 @@ static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
              launch_copy_rect<T>(D + (long long)H1 * Dp, Dp, A + (long long)H1 * p, p, W, H - H1, sc.side);
              if (L == 1) launch_copy_rect<T>(D, Dp, A, p, W1, H1, sc.side));
     }
+  }
+  if (jt < L) {
+    // levels jt .. L-1 in one single-workgroup launch; LL_jt is in P[jt % 2]
+    ProfScope ps(4, jt, st);
+    LAUNCH(launch_fwd_tail<T>(P[jt % 2], Pp[jt % 2], ceil_div_pow2(W, jt), ceil_div_pow2(H, jt),
+                              L - jt, A, p, cf, st));
   }
   HIPCHK(hipEventRecord(sc.join, sc.side));
   HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
@@ -372,7 +383,14 @@ static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
            if (L == 1) launch_copy_rect<T>(A, p, D, Dp, W1, H1, sc.side));
   }
   HIPCHK(hipEventRecord(sc.join, sc.side));
-  for (int j = L; j >= 1; --j) {
+  const int jt = tail_start(W, H, L, (int)sizeof(T));
+  if (jt < L) {
+    // inverse levels L .. jt+1 in one single-workgroup launch; LL_jt to P[jt % 2]
+    ProfScope ps(5, jt, st);
+    LAUNCH(launch_inv_tail<T>(A, p, ceil_div_pow2(W, jt), ceil_div_pow2(H, jt), L - jt,
+                              P[jt % 2], Pp[jt % 2], cf, st));
+  }
+  for (int j = jt; j >= 1; --j) {
     const int Wf = ceil_div_pow2(W, j - 1), Hf = ceil_div_pow2(H, j - 1);
     const T* ll;
     long long llp;
@@ -812,6 +830,11 @@ int dwt_gpu_tile_get(int forward, int width, int height, int wavelet, int* seg_r
                      int* prefetch_pairs) {
   if (!seg_rows || !prefetch_pairs || width < 1 || height < 1) return DWT_GPU_EINVAL;
   return tile_get(forward, height, width, wavelet == DWT_GPU_CDF53 ? 2 : 4, seg_rows, prefetch_pairs);
+}
+
+int dwt_gpu_tail_set(long long max_input_bytes, int seg_rows) {
+  tail_config(max_input_bytes, seg_rows);
+  return DWT_GPU_OK;
 }
 
 long long dwt_gpu_scratch_bytes(void* stream) {

@@ -8,8 +8,8 @@
 // 4 for CDF 9/7, 2 for CDF 5/3).  The wave walks down a segment of S rows
 // (plus HALO rows above and below), two rows at a time:
 //   * horizontal lifting of each row runs in registers; the neighbour sample
-//     across a lane boundary comes from the adjacent lane (wavefront
-//     shuffle), so no LDS is used;
+//     across a lane boundary comes from the adjacent lane through a DPP
+//     wave shift, so no LDS is used;
 //   * vertical lifting is a per-lane sliding window: each new row pair
 //     advances every lifting step by one position and completes one output
 //     pair (L, H) of every column, so the column pass costs no extra memory
@@ -37,12 +37,33 @@ __device__ inline int xcd_remap(int b, int nb) {
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
-// Value held by lane+1 / lane-1 (the edge lanes get their own value; those
-// lanes are halo lanes whose results are never stored).
-template <typename T>
-__device__ inline T from_next_lane(T v) { return __shfl_down(v, 1, kWave); }
-template <typename T>
-__device__ inline T from_prev_lane(T v) { return __shfl_up(v, 1, kWave); }
+// Value held by lane+1 / lane-1 of the wavefront, as a DPP modifier on a
+// VALU move (wave_shl:1 / wave_shr:1) rather than an LDS-crossbar permute, so
+// a lifting step's neighbour costs a few cycles instead of an LDS round trip.
+// The edge lanes (0 and 63) receive their own value; they are halo lanes whose
+// results are never stored.
+__device__ inline int dpp_from_next(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xF, 0xF, false);  // wave_shl:1
+}
+__device__ inline int dpp_from_prev(int v) {
+  return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false);  // wave_shr:1
+}
+__device__ inline float from_next_lane(float v) {
+  return __int_as_float(dpp_from_next(__float_as_int(v)));
+}
+__device__ inline float from_prev_lane(float v) {
+  return __int_as_float(dpp_from_prev(__float_as_int(v)));
+}
+__device__ inline double from_next_lane(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_from_next((int)(b & 0xffffffffLL)), hi = dpp_from_next((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double from_prev_lane(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_from_prev((int)(b & 0xffffffffLL)), hi = dpp_from_prev((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 template <typename T> struct Vec2;
 template <> struct Vec2<float> { using type = float2; };
@@ -158,22 +179,26 @@ struct ColInv {
   }
 };
 
-template <typename T, int NS, bool VEC, int U>
-__global__ void __launch_bounds__(256)
-fwd_level_kernel(const T* __restrict__ src, long long sp, int W, int H, T* __restrict__ ll,
-                 long long llp, T* __restrict__ det, long long dp, int S, int nstrips, int nseg,
-                 Coeffs<T> cf) {
+// ---- level bodies -----------------------------------------------------------
+// One wavefront computes one tile: the core columns of strip `strip` and the
+// rows of segment `seg` (S rows).  Where rows come from and where the LL
+// coefficients go are policies (global or LDS), so that the per-level kernels
+// and the single-workgroup tail kernel share the arithmetic.  Each policy
+// addresses memory through a pointer of one known address space.
+//
+// LoadRow(Y, Xe, xe, xo, interior, e, o): samples (Xe, Xe + 1) of row Y of the
+//   level input, Y possibly outside [0, H) (the policy mirrors it); xe / xo are
+//   the mirrored column indices, `interior` says the wave's whole window lies
+//   inside the row.
+// StoreLL(m, k, v): LL coefficient (row m, column k) of the level output.
+template <typename T, int NS, int U, class LoadRow, class StoreLL>
+__device__ __forceinline__ void fwd_tile(int strip, int seg, int S, int W, int H,
+                                         const LoadRow& load_row, const StoreLL& store_ll,
+                                         T* __restrict__ det, long long dp, const Coeffs<T>& cf) {
   constexpr int HALO = NS;
   constexpr int CORE = 2 * kWave - 2 * HALO;
   constexpr int LAG = NS / 2;
-  using V2 = typename Vec2<T>::type;
-
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int g = b / nseg, s = b - g * nseg;
   const int lane = threadIdx.x & (kWave - 1);
-  const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
-  if (strip >= nstrips) return;
-
   const int x0 = strip * CORE;
   const int Xe = x0 - HALO + 2 * lane;
   const int xe = mirror(Xe, W), xo = mirror(Xe + 1, W);
@@ -182,34 +207,22 @@ fwd_level_kernel(const T* __restrict__ src, long long sp, int W, int H, T* __res
   const bool core = lane >= HALO / 2 && lane < kWave - HALO / 2;
   const bool st_a = core && k < Wd;
   const bool st_b = core && k < Wh;
-  const bool vec = VEC && (x0 - HALO >= 0) && (x0 - HALO + 2 * kWave <= W);
+  const bool interior = (x0 - HALO >= 0) && (x0 - HALO + 2 * kWave <= W);
 
-  const int m0 = (s * S) >> 1;
-  const int m1 = min((s * S + S) >> 1, Hd);
+  const int m0 = (seg * S) >> 1;
+  const int m1 = min((seg * S + S) >> 1, Hd);
   const int pbeg = m0 - LAG, pend = m1 + LAG - 1;
 
   ColFwd<T, NS> ca, cb;
   T ce[2 * U], co[2 * U];
-
-  auto load_row = [&](int Y, T& e, T& o) {
-    const T* rp = src + (long long)mirror(Y, H) * sp;
-    if (vec) {
-      const V2 v = *reinterpret_cast<const V2*>(rp + Xe);
-      e = v.x;
-      o = v.y;
-    } else {
-      e = rp[xe];
-      o = rp[xo];
-    }
-  };
-
 #pragma unroll
-  for (int u = 0; u < 2 * U; ++u) load_row(2 * pbeg + u, ce[u], co[u]);
+  for (int u = 0; u < 2 * U; ++u) load_row(2 * pbeg + u, Xe, xe, xo, interior, ce[u], co[u]);
 
   for (int pb = pbeg; pb <= pend; pb += U) {
     T ne[2 * U], no[2 * U];
 #pragma unroll
-    for (int u = 0; u < 2 * U; ++u) load_row(2 * (pb + U) + u, ne[u], no[u]);
+    for (int u = 0; u < 2 * U; ++u)
+      load_row(2 * (pb + U) + u, Xe, xe, xo, interior, ne[u], no[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       T e0 = ce[2 * u], o0 = co[2 * u], e1 = ce[2 * u + 1], o1 = co[2 * u + 1];
@@ -221,7 +234,7 @@ fwd_level_kernel(const T* __restrict__ src, long long sp, int W, int H, T* __res
       const int m = pb + u - LAG;
       if (m >= m0 && m < m1) {
         if (st_a) {
-          ll[(long long)m * llp + k] = La;
+          store_ll(m, k, La);
           if (m < Hh) det[(long long)(Hd + m) * dp + k] = Ha;
         }
         if (st_b) {
@@ -238,22 +251,19 @@ fwd_level_kernel(const T* __restrict__ src, long long sp, int W, int H, T* __res
   }
 }
 
-template <typename T, int NS, bool VEC, int U>
-__global__ void __launch_bounds__(256)
-inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ det, long long dp,
-                 T* __restrict__ dst, long long dstp, int W, int H, int S, int nstrips, int nseg,
-                 Coeffs<T> cf) {
+// Inverse tile: output columns of strip `strip`, rows of segment `seg` of the
+// W x H fine region.  LoadLL(m, k): LL coefficient; details come from det at
+// their Mallat positions.  StoreOut(r, Xe, e, o, st_e, st_o): output samples
+// (Xe, Xe + 1) of row r.
+template <typename T, int NS, int U, class LoadLL, class StoreOut>
+__device__ __forceinline__ void inv_tile(int strip, int seg, int S, int W, int H,
+                                         const LoadLL& load_ll, const T* __restrict__ det,
+                                         long long dp, const StoreOut& store_out,
+                                         const Coeffs<T>& cf) {
   constexpr int HALO = NS;
   constexpr int CORE = 2 * kWave - 2 * HALO;
   constexpr int LAG = NS / 2;
-  using V2 = typename Vec2<T>::type;
-
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int g = b / nseg, s = b - g * nseg;
   const int lane = threadIdx.x & (kWave - 1);
-  const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
-  if (strip >= nstrips) return;
-
   const int x0 = strip * CORE;
   const int Xe = x0 - HALO + 2 * lane;
   const int ke = mirror(Xe, W) >> 1, ko = mirror(Xe + 1, W) >> 1;
@@ -261,9 +271,8 @@ inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ 
   const bool core = lane >= HALO / 2 && lane < kWave - HALO / 2;
   const bool st_e = core && Xe < W;
   const bool st_o = core && Xe + 1 < W;
-  const bool vec = VEC && st_o;
 
-  const int y0 = s * S;
+  const int y0 = seg * S;
   const int y1 = min(y0 + S, H);
   const int pbeg = (y0 >> 1) - LAG, pend = (y1 >> 1) + LAG + 1;
 
@@ -278,21 +287,8 @@ inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ 
       e = r[ke];
       o = r[Wc + ko];
     } else {
-      e = ll[(long long)m * llp + ke];
+      e = load_ll(m, ke);
       o = det[(long long)m * dp + Wc + ko];
-    }
-  };
-  auto store_row = [&](int r, T e, T o) {
-    if (r < y0 || r >= y1) return;
-    T* rp = dst + (long long)r * dstp + Xe;
-    if (vec) {
-      V2 v;
-      v.x = e;
-      v.y = o;
-      *reinterpret_cast<V2*>(rp) = v;
-    } else {
-      if (st_e) rp[0] = e;
-      if (st_o) rp[1] = o;
     }
   };
 
@@ -311,11 +307,10 @@ inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ 
       T fe, se_, fo, so_;
       cle.push(e0, e1, cf, fe, se_);
       clo.push(o0, o1, cf, fo, so_);
-      const int p = pb + u;
-      const int r0 = NS == 4 ? 2 * p - 3 : 2 * p - 2;
+      const int r0 = NS == 4 ? 2 * (pb + u) - 3 : 2 * (pb + u) - 2;
       if (core) {
-        store_row(r0, fe, fo);
-        store_row(r0 + 1, se_, so_);
+        if (r0 >= y0 && r0 < y1) store_out(r0, Xe, fe, fo, st_e, st_o);
+        if (r0 + 1 >= y0 && r0 + 1 < y1) store_out(r0 + 1, Xe, se_, so_, st_e, st_o);
       }
     }
 #pragma unroll
@@ -323,6 +318,205 @@ inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ 
       ce[u] = ne[u];
       co[u] = no[u];
     }
+  }
+}
+
+// ---- row sources and destinations ------------------------------------------------
+template <typename T, bool VEC>
+struct GlobalRows {  // packed rows in global memory
+  const T* __restrict__ p;
+  long long pitch;
+  int H;
+  __device__ __forceinline__ void operator()(int Y, int Xe, int xe, int xo, bool interior, T& e,
+                                             T& o) const {
+    using V2 = typename Vec2<T>::type;
+    const T* rp = p + (long long)mirror(Y, H) * pitch;
+    if (VEC && interior) {
+      const V2 v = *reinterpret_cast<const V2*>(rp + Xe);
+      e = v.x;
+      o = v.y;
+    } else {
+      e = rp[xe];
+      o = rp[xo];
+    }
+  }
+};
+// LDS policies hold pointers typed in the LDS address space, so every access
+// through them is a ds_* instruction: a generic pointer would let the compiler
+// fold an LDS load and a global load of the two arms of a branch into one flat
+// load, and flat access to LDS is what the codegen guard forbids.
+template <typename T>
+using Lds = __attribute__((address_space(3))) T;
+
+template <typename T>
+struct LdsRows {  // packed rows in LDS (pitch = W)
+  const Lds<T>* p;
+  int pitch, H;
+  __device__ __forceinline__ void operator()(int Y, int, int xe, int xo, bool, T& e, T& o) const {
+    const Lds<T>* rp = p + mirror(Y, H) * pitch;
+    e = rp[xe];
+    o = rp[xo];
+  }
+};
+template <typename T>
+struct GlobalStore {  // element store, global
+  T* __restrict__ p;
+  long long pitch;
+  __device__ __forceinline__ void operator()(int m, int k, T v) const { p[(long long)m * pitch + k] = v; }
+};
+template <typename T>
+struct LdsStore {
+  Lds<T>* p;
+  int pitch;
+  __device__ __forceinline__ void operator()(int m, int k, T v) const { p[m * pitch + k] = v; }
+};
+template <typename T>
+struct GlobalLoad {
+  const T* __restrict__ p;
+  long long pitch;
+  __device__ __forceinline__ T operator()(int m, int k) const { return p[(long long)m * pitch + k]; }
+};
+template <typename T>
+struct LdsLoad {
+  const Lds<T>* p;
+  int pitch;
+  __device__ __forceinline__ T operator()(int m, int k) const { return p[m * pitch + k]; }
+};
+template <typename T, bool VEC>
+struct GlobalOut {  // two adjacent output samples, global
+  T* __restrict__ p;
+  long long pitch;
+  __device__ __forceinline__ void operator()(int r, int Xe, T e, T o, bool st_e, bool st_o) const {
+    using V2 = typename Vec2<T>::type;
+    T* rp = p + (long long)r * pitch + Xe;
+    if (VEC && st_o) {
+      V2 v;
+      v.x = e;
+      v.y = o;
+      *reinterpret_cast<V2*>(rp) = v;
+    } else {
+      if (st_e) rp[0] = e;
+      if (st_o) rp[1] = o;
+    }
+  }
+};
+template <typename T>
+struct LdsOut {
+  Lds<T>* p;
+  int pitch;
+  __device__ __forceinline__ void operator()(int r, int Xe, T e, T o, bool st_e, bool st_o) const {
+    Lds<T>* rp = p + r * pitch + Xe;
+    if (st_e) rp[0] = e;
+    if (st_o) rp[1] = o;
+  }
+};
+
+// ---- per-level kernels (many workgroups) -------------------------------------
+template <typename T, int NS, bool VEC, int U>
+__global__ void __launch_bounds__(256)
+fwd_level_kernel(const T* __restrict__ src, long long sp, int W, int H, T* __restrict__ ll,
+                 long long llp, T* __restrict__ det, long long dp, int S, int nstrips, int nseg,
+                 Coeffs<T> cf) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int g = b / nseg, s = b - g * nseg;
+  const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
+  if (strip >= nstrips) return;
+  fwd_tile<T, NS, U>(strip, s, S, W, H, GlobalRows<T, VEC>{src, sp, H}, GlobalStore<T>{ll, llp},
+                     det, dp, cf);
+}
+
+template <typename T, int NS, bool VEC, int U>
+__global__ void __launch_bounds__(256)
+inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ det, long long dp,
+                 T* __restrict__ dst, long long dstp, int W, int H, int S, int nstrips, int nseg,
+                 Coeffs<T> cf) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int g = b / nseg, s = b - g * nseg;
+  const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
+  if (strip >= nstrips) return;
+  inv_tile<T, NS, U>(strip, s, S, W, H, GlobalLoad<T>{ll, llp}, det, dp,
+                     GlobalOut<T, VEC>{dst, dstp}, cf);
+}
+
+// ---- tail kernels (one workgroup, several small levels) --------------------------
+// The small levels of a transform are latency-bound: each kernel boundary on
+// this multi-XCD GPU writes back and invalidates L2 and the next level starts
+// from cold caches.  One workgroup of 16 wavefronts runs all of them in one
+// launch instead: the LL band of each level stays in LDS (two ping-pong
+// buffers) and only the detail subbands go to global memory.  The first tail
+// level reads its input from global memory, the last writes LL_J to global.
+constexpr int kTailThreads = 1024;
+
+template <typename T, int NS, int U>
+__global__ void __launch_bounds__(kTailThreads)
+fwd_tail_kernel(const T* __restrict__ src, long long sp, int W0, int H0, int nlev,
+                T* __restrict__ A, long long ap, int S, int buf1_off, Coeffs<T> cf) {
+  // both LDS buffers are one LDS-typed base plus an element offset
+  extern __shared__ __align__(16) unsigned char tail_smem[];
+  Lds<T>* const lds = (Lds<T>*)tail_smem;  // generic -> LDS address-space cast
+  constexpr int CORE = 2 * kWave - 2 * NS;
+  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  for (int lev = 0; lev < nlev; ++lev) {
+    const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
+    const int Wd = (W + 1) >> 1;
+    const int nstrips = (W + CORE - 1) / CORE, nseg = (H + S - 1) / S;
+    const bool last = lev == nlev - 1;
+    for (int t = wave; t < nstrips * nseg; t += nwaves) {
+      const int strip = t / nseg, seg = t - strip * nseg;
+      if (lev == 0) {
+        const GlobalRows<T, false> rows{src, sp, H};
+        if (last)
+          fwd_tile<T, NS, U>(strip, seg, S, W, H, rows, GlobalStore<T>{A, ap}, A, ap, cf);
+        else
+          fwd_tile<T, NS, U>(strip, seg, S, W, H, rows, LdsStore<T>{lds, Wd}, A, ap, cf);
+      } else {
+        const LdsRows<T> rows{lds + ((lev - 1) & 1) * buf1_off, W, H};
+        if (last)
+          fwd_tile<T, NS, U>(strip, seg, S, W, H, rows, GlobalStore<T>{A, ap}, A, ap, cf);
+        else
+          fwd_tile<T, NS, U>(strip, seg, S, W, H, rows, LdsStore<T>{lds + (lev & 1) * buf1_off, Wd},
+                             A, ap, cf);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Inverse tail: levels nlev-1 .. 0 of the region whose level-0 (fine) size is
+// W0 x H0.  LL of the coarsest level comes from A; the output of level 0 goes
+// to dst (pitch dstp), the outputs of the others to LDS.
+template <typename T, int NS, int U>
+__global__ void __launch_bounds__(kTailThreads)
+inv_tail_kernel(const T* __restrict__ A, long long ap, int W0, int H0, int nlev,
+                T* __restrict__ dst, long long dstp, int S, int buf1_off, Coeffs<T> cf) {
+  extern __shared__ __align__(16) unsigned char tail_smem[];
+  Lds<T>* const lds = (Lds<T>*)tail_smem;  // generic -> LDS address-space cast
+  constexpr int CORE = 2 * kWave - 2 * NS;
+  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+  for (int lev = nlev - 1; lev >= 0; --lev) {
+    const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
+    const int Wc = (W + 1) >> 1;
+    const int nstrips = (W + CORE - 1) / CORE, nseg = (H + S - 1) / S;
+    const bool first = lev == nlev - 1;
+    for (int t = wave; t < nstrips * nseg; t += nwaves) {
+      const int strip = t / nseg, seg = t - strip * nseg;
+      if (first) {
+        const GlobalLoad<T> llsrc{A, ap};
+        if (lev == 0)
+          inv_tile<T, NS, U>(strip, seg, S, W, H, llsrc, A, ap, GlobalOut<T, false>{dst, dstp}, cf);
+        else
+          inv_tile<T, NS, U>(strip, seg, S, W, H, llsrc, A, ap,
+                             LdsOut<T>{lds + (lev & 1) * buf1_off, W}, cf);
+      } else {
+        const LdsLoad<T> llsrc{lds + ((lev + 1) & 1) * buf1_off, Wc};
+        if (lev == 0)
+          inv_tile<T, NS, U>(strip, seg, S, W, H, llsrc, A, ap, GlobalOut<T, false>{dst, dstp}, cf);
+        else
+          inv_tile<T, NS, U>(strip, seg, S, W, H, llsrc, A, ap,
+                             LdsOut<T>{lds + (lev & 1) * buf1_off, W}, cf);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -436,6 +630,94 @@ void launch_inv_level(const T* ll, long long llp, const T* det, long long dp, T*
     default: DWT_DISPATCH_INV(4, true, 8); break;
   }
 }
+
+// ---- tail launchers --------------------------------------------------------------
+// A tail starts at level jt >= 1 (level 0 is in place on the caller's image and
+// always runs wide) once the level input is at most g_tail_bytes, provided the
+// two LDS buffers for its LL outputs fit.
+static long long g_tail_bytes = 256 * 1024;
+static int g_tail_seg_rows = 8;
+constexpr int kTailLdsMax = 150 * 1024;
+
+void tail_config(long long max_input_bytes, int seg_rows) {
+  g_tail_bytes = max_input_bytes;
+  if (seg_rows >= 2 && !(seg_rows & 1)) g_tail_seg_rows = seg_rows;
+}
+
+int tail_lds_bytes(int W0, int H0, int elem) {
+  const long long w1 = ceil_div_pow2(W0, 1), h1 = ceil_div_pow2(H0, 1);
+  const long long w2 = ceil_div_pow2(W0, 2), h2 = ceil_div_pow2(H0, 2);
+  const long long b = (w1 * h1 + w2 * h2) * elem;
+  return b > (1 << 30) ? (1 << 30) : (int)b;
+}
+
+int tail_start(int W, int H, int L, int elem) {
+  if (g_tail_bytes <= 0) return L;
+  for (int j = 1; j < L; ++j) {
+    const long long wj = ceil_div_pow2(W, j), hj = ceil_div_pow2(H, j);
+    if (wj * hj * elem <= g_tail_bytes &&
+        tail_lds_bytes((int)wj, (int)hj, elem) <= kTailLdsMax)
+      return j;
+  }
+  return L;
+}
+
+hipError_t configure_tail_kernels() {
+  const void* fns[] = {
+      reinterpret_cast<const void*>(&fwd_tail_kernel<float, 4, 4>),
+      reinterpret_cast<const void*>(&fwd_tail_kernel<float, 2, 4>),
+      reinterpret_cast<const void*>(&fwd_tail_kernel<double, 4, 4>),
+      reinterpret_cast<const void*>(&fwd_tail_kernel<double, 2, 4>),
+      reinterpret_cast<const void*>(&inv_tail_kernel<float, 4, 4>),
+      reinterpret_cast<const void*>(&inv_tail_kernel<float, 2, 4>),
+      reinterpret_cast<const void*>(&inv_tail_kernel<double, 4, 4>),
+      reinterpret_cast<const void*>(&inv_tail_kernel<double, 2, 4>)};
+  for (const void* f : fns) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsMax);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return e;
+    }
+  }
+  return hipSuccess;
+}
+
+template <typename T>
+void launch_fwd_tail(const T* src, long long sp, int W0, int H0, int nlev, T* A, long long ap,
+                     const Coeffs<T>& cf, hipStream_t st) {
+  const long long w1 = ceil_div_pow2(W0, 1), h1 = ceil_div_pow2(H0, 1);
+  const int lds = tail_lds_bytes(W0, H0, sizeof(T));
+  const int off = (int)(w1 * h1);
+  if (cf.nsteps == 4)
+    hipLaunchKernelGGL((fwd_tail_kernel<T, 4, 4>), dim3(1), dim3(kTailThreads), lds, st, src, sp,
+                       W0, H0, nlev, A, ap, g_tail_seg_rows, off, cf);
+  else
+    hipLaunchKernelGGL((fwd_tail_kernel<T, 2, 4>), dim3(1), dim3(kTailThreads), lds, st, src, sp,
+                       W0, H0, nlev, A, ap, g_tail_seg_rows, off, cf);
+}
+
+template <typename T>
+void launch_inv_tail(const T* A, long long ap, int W0, int H0, int nlev, T* dst, long long dstp,
+                     const Coeffs<T>& cf, hipStream_t st) {
+  const long long w2 = ceil_div_pow2(W0, 2), h2 = ceil_div_pow2(H0, 2);
+  const int lds = tail_lds_bytes(W0, H0, sizeof(T));
+  const int off = (int)(w2 * h2);
+  if (cf.nsteps == 4)
+    hipLaunchKernelGGL((inv_tail_kernel<T, 4, 4>), dim3(1), dim3(kTailThreads), lds, st, A, ap,
+                       W0, H0, nlev, dst, dstp, g_tail_seg_rows, off, cf);
+  else
+    hipLaunchKernelGGL((inv_tail_kernel<T, 2, 4>), dim3(1), dim3(kTailThreads), lds, st, A, ap,
+                       W0, H0, nlev, dst, dstp, g_tail_seg_rows, off, cf);
+}
+
+template void launch_fwd_tail<float>(const float*, long long, int, int, int, float*, long long,
+                                     const Coeffs<float>&, hipStream_t);
+template void launch_fwd_tail<double>(const double*, long long, int, int, int, double*, long long,
+                                      const Coeffs<double>&, hipStream_t);
+template void launch_inv_tail<float>(const float*, long long, int, int, int, float*, long long,
+                                     const Coeffs<float>&, hipStream_t);
+template void launch_inv_tail<double>(const double*, long long, int, int, int, double*, long long,
+                                      const Coeffs<double>&, hipStream_t);
 
 template <typename T>
 void launch_copy_rect(const T* src, long long sp, T* dst, long long dp, int w, int h,
