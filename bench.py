@@ -142,8 +142,8 @@ def main():
     torch.cuda.synchronize()
     libdwt_amd.profile_enable(False)
     kernels = {}
-    for kind, name in ((0, "fwd_level"), (1, "inv_level"), (2, "detail_copy"), (4, "fwd_tail"),
-                       (5, "inv_tail")):
+    for kind, name in ((0, "fwd_level"), (1, "inv_level"), (2, "detail_copy"), (4, "fwd_block"),
+                       (5, "inv_block")):
         for lev in range(J):
             n, ms = libdwt_amd.profile_read(kind, lev)
             if n:

@@ -134,8 +134,9 @@ int dwt_gpu_2d_async(int wavelet, int direction, int precision, void *ptr, int s
 
 /* Per-launch timing with HIP events recorded on the launch stream around each
  * kernel of the packed path.  kind: 0 forward level kernel, 1 inverse level
- * kernel, 2 level-0 detail copy, 4 forward tail, 5 inverse tail; level: the
- * level index, the first tail level for a tail (-1 = all).
+ * kernel, 2 level-0 detail copy, 4 forward block kernel, 5 inverse block
+ * kernel; level: the level index, the first level it does for a block kernel
+ * (-1 = all).
  * profile_read synchronises on the recorded events. */
 int dwt_gpu_profile_enable(int on);
 int dwt_gpu_profile_read(int kind, int level, long long *launches, double *total_ms);
@@ -150,11 +151,11 @@ int dwt_gpu_tile_set(int forward, int height_class, int seg_rows, int prefetch_p
 int dwt_gpu_tile_get(int forward, int width, int height, int wavelet, int *seg_rows,
                      int *prefetch_pairs);
 
-/* Small levels run in one single-workgroup "tail" launch (LL bands in LDS)
- * from the first level >= 1 whose input is at most max_input_bytes
- * (default 256 KiB; 0 disables the tail); seg_rows is the tail's segment
- * height (even, default 8). */
-int dwt_gpu_tail_set(long long max_input_bytes, int seg_rows);
+/* The small levels of a transform run as one single-workgroup launch that
+ * keeps the whole small pyramid in LDS, from the first level whose region is
+ * at most max_region_bytes (default 64 KiB, at most 75 KiB; 0 disables it).
+ * An image that small is transformed completely by that one launch. */
+int dwt_gpu_tail_set(long long max_region_bytes);
 
 /* Bytes of device scratch the calling thread currently holds for `stream`. */
 long long dwt_gpu_scratch_bytes(void *stream);

@@ -83,7 +83,7 @@ def lib() -> C.CDLL:
         L.dwt_gpu_tile_set.restype = i
         L.dwt_gpu_tile_get.argtypes = [i, i, i, i, C.POINTER(i), C.POINTER(i)]
         L.dwt_gpu_tile_get.restype = i
-        L.dwt_gpu_tail_set.argtypes = [C.c_longlong, i]
+        L.dwt_gpu_tail_set.argtypes = [C.c_longlong]
         L.dwt_gpu_tail_set.restype = i
         L.dwt_gpu_scratch_bytes.argtypes = [vp]
         L.dwt_gpu_scratch_bytes.restype = C.c_longlong
@@ -148,8 +148,8 @@ def profile_reset() -> None:
 
 def profile_read(kind: int, level: int = -1) -> tuple[int, float]:
     """(launches, total milliseconds) of the recorded launches of one kind
-    (0 forward level, 1 inverse level, 2 detail copy, 4 forward tail,
-    5 inverse tail) at one level (-1 all)."""
+    (0 forward level, 1 inverse level, 2 detail copy, 4 forward block kernel,
+    5 inverse block kernel) at one level (-1 all)."""
     n = C.c_longlong(0)
     ms = C.c_double(0)
     _check(lib().dwt_gpu_profile_read(kind, level, C.byref(n), C.byref(ms)))

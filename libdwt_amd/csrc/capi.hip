@@ -210,8 +210,8 @@ static Scratch& scratch_for(hipStream_t st) {
 
 // ---- optional per-launch timing (HIP events on the launch stream) ---------------
 // Kinds: 0 forward level kernel, 1 inverse level kernel, 2 detail copy,
-// 4 forward tail kernel, 5 inverse tail kernel.  Level is the level index the
-// launch belongs to (the first tail level for a tail).
+// 4 forward block kernel, 5 inverse block kernel.  Level is the level index the
+// launch belongs to (the first level it does for a block kernel).
 struct ProfRec {
   int kind, level;
   hipEvent_t a, b;
@@ -299,14 +299,27 @@ static void generic_inv_level(char* ptr, long long sx, long long sy, int Wo, int
 }
 
 // ---- packed fast path -----------------------------------------------------------
-// A: W x H, row pitch p elements, levels [0, L) forward.  Level 0 reads A and
-// writes its detail subbands to scratch D (they overlap A's unread input);
-// level j >= 1 reads LL_j from ping-pong scratch P[j % 2] and writes its
-// details straight into A (that region of A was consumed by level 0); the
-// last level writes its LL into A; finally D's details are copied into A.
+// A: W x H, row pitch p elements, levels [0, L) forward.
+//  * Small levels: from level jt on (the first level whose W_jt x H_jt region
+//    fits one LDS buffer) a single block-kernel launch does all remaining
+//    levels inside one workgroup's LDS.  If jt == 0 the whole transform is
+//    that one launch (the block kernel reads its region before writing).
+//  * Level 0 (when jt > 0) reads A and writes its detail subbands to scratch D
+//    (they overlap A's unread input) and LL_1 to scratch P[1]; its details are
+//    copied into A on a side stream, overlapping the later levels, which touch
+//    disjoint memory.
+//  * Level 1 <= j < jt reads LL_j from ping-pong scratch P[j % 2] and writes
+//    its details straight into A (that part of A was consumed by level 0) and
+//    LL_{j+1} to P[(j + 1) % 2] (or to A when it is the last level).
 template <typename T>
 static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, Scratch& sc,
                      hipStream_t st) {
+  const int jt = tail_start(W, H, L, (int)sizeof(T));
+  if (jt == 0) {
+    ProfScope ps(4, 0, st);
+    LAUNCH(launch_block_fwd<T>(A, p, A, p, W, H, L, cf, st));
+    return;
+  }
   const int W1 = ceil_div_pow2(W, 1), H1 = ceil_div_pow2(H, 1);
   const int W2 = ceil_div_pow2(W, 2), H2 = ceil_div_pow2(H, 2);
   const long long Dp = (long long)pad_pitch(W);
@@ -316,7 +329,6 @@ static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
   if (L >= 2) P[1] = (T*)sc.get(1, sizeof(T) * (size_t)Pp[1] * H1);
   if (L >= 3) P[0] = (T*)sc.get(0, sizeof(T) * (size_t)Pp[0] * H2);
   sc.ensure_side();
-  const int jt = tail_start(W, H, L, (int)sizeof(T));
   for (int j = 0; j < jt; ++j) {
     const int Wj = ceil_div_pow2(W, j), Hj = ceil_div_pow2(H, j);
     const T* src = j == 0 ? A : P[j % 2];
@@ -338,7 +350,7 @@ static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
     }
     if (j == 0) {
       // details of level 0 (and its LL when it is the only level) into A, on
-      // the side stream while levels >= 1 run on st
+      // the side stream while the later levels run on st
       HIPCHK(hipEventRecord(sc.fork, st));
       HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
       ProfScope ps(2, 0, sc.side);
@@ -348,21 +360,29 @@ static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
     }
   }
   if (jt < L) {
-    // levels jt .. L-1 in one single-workgroup launch; LL_jt is in P[jt % 2]
+    // levels jt .. L-1 in one block launch: LL_jt from P[jt % 2], the region's
+    // coefficients into A
     ProfScope ps(4, jt, st);
-    LAUNCH(launch_fwd_tail<T>(P[jt % 2], Pp[jt % 2], ceil_div_pow2(W, jt), ceil_div_pow2(H, jt),
-                              L - jt, A, p, cf, st));
+    LAUNCH(launch_block_fwd<T>(P[jt % 2], Pp[jt % 2], A, p, ceil_div_pow2(W, jt),
+                               ceil_div_pow2(H, jt), L - jt, cf, st));
   }
   HIPCHK(hipEventRecord(sc.join, sc.side));
   HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
 }
 
-// Inverse of fast_fwd: LL_L is in A.  Level j >= 2 writes LL_{j-1} to P;
-// level 1 writes the image into A, so the level-0 details (and LL_1 when
-// L == 1) are first copied out of A into D.
+// Inverse of fast_fwd: the block kernel does levels L .. jt+1 (writing LL_jt
+// to P[jt % 2]); level j >= 2 writes LL_{j-1} to P; level 1 writes the image
+// into A, so the level-0 details (and LL_1 when L == 1) are first copied out of
+// A into D on the side stream.
 template <typename T>
 static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, Scratch& sc,
                      hipStream_t st) {
+  const int jt = tail_start(W, H, L, (int)sizeof(T));
+  if (jt == 0) {
+    ProfScope ps(5, 0, st);
+    LAUNCH(launch_block_inv<T>(A, p, A, p, W, H, L, cf, st));
+    return;
+  }
   const int W1 = ceil_div_pow2(W, 1), H1 = ceil_div_pow2(H, 1);
   const int W2 = ceil_div_pow2(W, 2), H2 = ceil_div_pow2(H, 2);
   const long long Dp = (long long)pad_pitch(W);
@@ -371,8 +391,6 @@ static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
   long long Pp[2] = {(long long)pad_pitch(W2), (long long)pad_pitch(W1)};
   if (L >= 2) P[1] = (T*)sc.get(1, sizeof(T) * (size_t)Pp[1] * H1);
   if (L >= 3) P[0] = (T*)sc.get(0, sizeof(T) * (size_t)Pp[0] * H2);
-  // copy the level-0 details (and LL_1 when it is the only level) out of A on
-  // the side stream while levels L..2 run; level 1 waits for it
   sc.ensure_side();
   HIPCHK(hipEventRecord(sc.fork, st));
   HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
@@ -383,12 +401,10 @@ static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
            if (L == 1) launch_copy_rect<T>(A, p, D, Dp, W1, H1, sc.side));
   }
   HIPCHK(hipEventRecord(sc.join, sc.side));
-  const int jt = tail_start(W, H, L, (int)sizeof(T));
   if (jt < L) {
-    // inverse levels L .. jt+1 in one single-workgroup launch; LL_jt to P[jt % 2]
     ProfScope ps(5, jt, st);
-    LAUNCH(launch_inv_tail<T>(A, p, ceil_div_pow2(W, jt), ceil_div_pow2(H, jt), L - jt,
-                              P[jt % 2], Pp[jt % 2], cf, st));
+    LAUNCH(launch_block_inv<T>(A, p, P[jt % 2], Pp[jt % 2], ceil_div_pow2(W, jt),
+                               ceil_div_pow2(H, jt), L - jt, cf, st));
   }
   for (int j = jt; j >= 1; --j) {
     const int Wf = ceil_div_pow2(W, j - 1), Hf = ceil_div_pow2(H, j - 1);
@@ -832,8 +848,8 @@ int dwt_gpu_tile_get(int forward, int width, int height, int wavelet, int* seg_r
   return tile_get(forward, height, width, wavelet == DWT_GPU_CDF53 ? 2 : 4, seg_rows, prefetch_pairs);
 }
 
-int dwt_gpu_tail_set(long long max_input_bytes, int seg_rows) {
-  tail_config(max_input_bytes, seg_rows);
+int dwt_gpu_tail_set(long long max_region_bytes) {
+  tail_config(max_region_bytes, 0);
   return DWT_GPU_OK;
 }
 

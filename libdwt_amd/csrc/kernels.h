@@ -35,20 +35,20 @@ void launch_inv_level(const T* ll, long long llp, const T* det, long long dp, T*
 // launch of that size would use.
 int tile_set(int fwd, int hclass, int S, int U);
 int tile_get(int fwd, int H, int W, int nsteps, int* S, int* U);
-// Tail: all levels from a small level on, in one single-workgroup launch with
-// the LL bands in LDS.  Forward: input W0 x H0 (the first tail level) from src,
-// nlev levels, details and the final LL into A (Mallat layout at A's origin).
-// Inverse: the nlev levels whose finest output is W0 x H0, LL and details from
-// A, output to dst.  tail_start picks the first tail level of an L-level
-// transform (L when no tail applies).
+// Block kernels: a whole small pyramid in one workgroup's LDS.  Forward:
+// levels 0..nlev-1 of the W0 x H0 region read from src, written in Mallat
+// layout to dst.  Inverse: the nlev levels whose finest output is W0 x H0,
+// coefficients from src, output to dst.  src may equal dst (the region is read
+// completely before it is written).  tail_start picks the first level of an
+// L-level transform that the block kernel takes (L when none).
 template <typename T>
-void launch_fwd_tail(const T* src, long long sp, int W0, int H0, int nlev, T* A, long long ap,
-                     const Coeffs<T>& cf, hipStream_t st);
+void launch_block_fwd(const T* src, long long sp, T* dst, long long dp, int W0, int H0, int nlev,
+                      const Coeffs<T>& cf, hipStream_t st);
 template <typename T>
-void launch_inv_tail(const T* A, long long ap, int W0, int H0, int nlev, T* dst, long long dstp,
-                     const Coeffs<T>& cf, hipStream_t st);
+void launch_block_inv(const T* src, long long sp, T* dst, long long dp, int W0, int H0, int nlev,
+                      const Coeffs<T>& cf, hipStream_t st);
 int tail_start(int W, int H, int L, int elem);
-void tail_config(long long max_input_bytes, int seg_rows);
+void tail_config(long long max_region_bytes, int unused);
 hipError_t configure_tail_kernels();
 // Copy a w x h rectangle between packed buffers.
 template <typename T>

@@ -438,85 +438,138 @@ inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ 
                      GlobalOut<T, VEC>{dst, dstp}, cf);
 }
 
-// ---- tail kernels (one workgroup, several small levels) --------------------------
-// The small levels of a transform are latency-bound: each kernel boundary on
-// this multi-XCD GPU writes back and invalidates L2 and the next level starts
-// from cold caches.  One workgroup of 16 wavefronts runs all of them in one
-// launch instead: the LL band of each level stays in LDS (two ping-pong
-// buffers) and only the detail subbands go to global memory.  The first tail
-// level reads its input from global memory, the last writes LL_J to global.
-constexpr int kTailThreads = 1024;
+// ---- block kernels (one workgroup, a whole small pyramid in LDS) --------------
+// The small levels of a transform are latency-bound: every kernel boundary on
+// this multi-XCD GPU writes back and invalidates L2, and a level of a few KB
+// cannot fill the machine.  A block kernel instead moves the whole small Mallat
+// region (the LL input of the first small level, or all coefficients of the
+// small levels for the inverse) into LDS with one cooperative sweep, runs every
+// remaining level there, and writes the region back with one sweep: two global
+// round trips for all small levels.  Because the whole region is read before
+// anything is written, it may also work in place (an image small enough to fit
+// is transformed completely by one launch).
+//
+// Within LDS a level is computed line-parallel by the whole workgroup, each
+// lifting step a barrier-separated sweep (x + c * (l + r), a missing neighbour
+// at a line end replaced by the existing one), then the samples are
+// deinterleaved into Mallat order through the second LDS buffer.  Rows first,
+// then columns, in both directions.
+constexpr int kBlockThreads = 1024;
 
-template <typename T, int NS, int U>
-__global__ void __launch_bounds__(kTailThreads)
-fwd_tail_kernel(const T* __restrict__ src, long long sp, int W0, int H0, int nlev,
-                T* __restrict__ A, long long ap, int S, int buf1_off, Coeffs<T> cf) {
-  // both LDS buffers are one LDS-typed base plus an element offset
-  extern __shared__ __align__(16) unsigned char tail_smem[];
-  Lds<T>* const lds = (Lds<T>*)tail_smem;  // generic -> LDS address-space cast
-  constexpr int CORE = 2 * kWave - 2 * NS;
-  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
-  for (int lev = 0; lev < nlev; ++lev) {
-    const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
-    const int Wd = (W + 1) >> 1;
-    const int nstrips = (W + CORE - 1) / CORE, nseg = (H + S - 1) / S;
-    const bool last = lev == nlev - 1;
-    for (int t = wave; t < nstrips * nseg; t += nwaves) {
-      const int strip = t / nseg, seg = t - strip * nseg;
-      if (lev == 0) {
-        const GlobalRows<T, false> rows{src, sp, H};
-        if (last)
-          fwd_tile<T, NS, U>(strip, seg, S, W, H, rows, GlobalStore<T>{A, ap}, A, ap, cf);
-        else
-          fwd_tile<T, NS, U>(strip, seg, S, W, H, rows, LdsStore<T>{lds, Wd}, A, ap, cf);
+template <typename T>
+__device__ __forceinline__ void lds_lift_pass(Lds<T>* buf, int pitch, int nlines, int N,
+                                              bool rows, const Coeffs<T>& cf) {
+  // lifting steps on nlines lines of length N >= 2 held interleaved in buf
+  // (rows: line y is buf[y*pitch + i]; columns: line x is buf[i*pitch + x])
+  for (int st = 0; st < cf.nsteps; ++st) {
+    const int par = (cf.first_parity + st) & 1;
+    const int cnt = (N - par + 1) >> 1;  // samples of this parity
+    const T c = cf.c[st];
+    const T c2 = T(2) * c;
+    for (int w = threadIdx.x; w < nlines * cnt; w += blockDim.x) {
+      const int line = w / cnt, i = par + 2 * (w - line * cnt);
+      const int base = rows ? line * pitch : line;
+      const int step = rows ? 1 : pitch;
+      Lds<T>* x = buf + base + i * step;
+      T v;
+      if (i > 0 && i + 1 < N) {
+        T t = x[-step] + x[step];
+        t = c * t;
+        v = x[0] + t;
       } else {
-        const LdsRows<T> rows{lds + ((lev - 1) & 1) * buf1_off, W, H};
-        if (last)
-          fwd_tile<T, NS, U>(strip, seg, S, W, H, rows, GlobalStore<T>{A, ap}, A, ap, cf);
-        else
-          fwd_tile<T, NS, U>(strip, seg, S, W, H, rows, LdsStore<T>{lds + (lev & 1) * buf1_off, Wd},
-                             A, ap, cf);
+        T t = c2 * (i > 0 ? x[-step] : x[step]);
+        v = x[0] + t;
       }
+      x[0] = v;
     }
     __syncthreads();
   }
 }
 
-// Inverse tail: levels nlev-1 .. 0 of the region whose level-0 (fine) size is
-// W0 x H0.  LL of the coarsest level comes from A; the output of level 0 goes
-// to dst (pitch dstp), the outputs of the others to LDS.
-template <typename T, int NS, int U>
-__global__ void __launch_bounds__(kTailThreads)
-inv_tail_kernel(const T* __restrict__ A, long long ap, int W0, int H0, int nlev,
-                T* __restrict__ dst, long long dstp, int S, int buf1_off, Coeffs<T> cf) {
-  extern __shared__ __align__(16) unsigned char tail_smem[];
-  Lds<T>* const lds = (Lds<T>*)tail_smem;  // generic -> LDS address-space cast
-  constexpr int CORE = 2 * kWave - 2 * NS;
-  const int wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+template <typename T>
+__device__ __forceinline__ void lds_scale_pass(Lds<T>* buf, int pitch, int W, int H, bool rows,
+                                               T se, T so) {
+  for (int w = threadIdx.x; w < W * H; w += blockDim.x) {
+    const int y = w / W, x = w - y * W;
+    const int par = rows ? (x & 1) : (y & 1);
+    buf[y * pitch + x] = buf[y * pitch + x] * (par ? so : se);
+  }
+  __syncthreads();
+}
+
+// out = permutation of in over the W x H region: forward deinterleave (even
+// samples to [0, ceil(N/2)), odd to [ceil(N/2), N)) or inverse interleave, along
+// rows or columns.
+template <typename T>
+__device__ __forceinline__ void lds_permute(const Lds<T>* in, Lds<T>* out, int pitch, int W, int H,
+                                            bool rows, bool deinterleave) {
+  const int N = rows ? W : H, half = (N + 1) >> 1;
+  for (int w = threadIdx.x; w < W * H; w += blockDim.x) {
+    const int y = w / W, x = w - y * W;
+    const int i = rows ? x : y;
+    const int j = deinterleave ? ((i & 1) ? half + (i >> 1) : (i >> 1))
+                               : (i < half ? 2 * i : 2 * (i - half) + 1);
+    const int yo = rows ? y : j, xo = rows ? j : x;
+    out[yo * pitch + xo] = in[y * pitch + x];
+  }
+  __syncthreads();
+}
+
+// Forward: levels 0..nlev-1 of the W0 x H0 region read from src (pitch sp),
+// result (Mallat layout) to dst (pitch dp); src may equal dst.
+template <typename T>
+__global__ void __launch_bounds__(kBlockThreads)
+block_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, long long dp, int W0,
+                 int H0, int nlev, Coeffs<T> cf) {
+  extern __shared__ __align__(16) unsigned char blk_smem[];
+  Lds<T>* const b0 = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
+  Lds<T>* const b1 = b0 + W0 * H0;
+  for (int w = threadIdx.x; w < W0 * H0; w += blockDim.x) {
+    const int y = w / W0, x = w - y * W0;
+    b0[w] = src[(long long)y * sp + x];
+  }
+  __syncthreads();
+  for (int lev = 0; lev < nlev; ++lev) {
+    const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
+    lds_lift_pass(b0, W0, H, W, true, cf);           // rows
+    lds_scale_pass(b0, W0, W, H, true, cf.se, cf.so);
+    lds_permute(b0, b1, W0, W, H, true, true);
+    lds_lift_pass(b1, W0, W, H, false, cf);          // columns
+    lds_scale_pass(b1, W0, W, H, false, cf.se, cf.so);
+    lds_permute(b1, b0, W0, W, H, false, true);
+  }
+  for (int w = threadIdx.x; w < W0 * H0; w += blockDim.x) {
+    const int y = w / W0, x = w - y * W0;
+    dst[(long long)y * dp + x] = b0[w];
+  }
+}
+
+// Inverse: the nlev levels whose finest output is W0 x H0; coefficients of the
+// W0 x H0 region from src (pitch sp), output to dst (pitch dp); src may equal dst.
+template <typename T>
+__global__ void __launch_bounds__(kBlockThreads)
+block_inv_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, long long dp, int W0,
+                 int H0, int nlev, Coeffs<T> cf) {
+  extern __shared__ __align__(16) unsigned char blk_smem[];
+  Lds<T>* const b0 = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
+  Lds<T>* const b1 = b0 + W0 * H0;
+  for (int w = threadIdx.x; w < W0 * H0; w += blockDim.x) {
+    const int y = w / W0, x = w - y * W0;
+    b0[w] = src[(long long)y * sp + x];
+  }
+  __syncthreads();
   for (int lev = nlev - 1; lev >= 0; --lev) {
     const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
-    const int Wc = (W + 1) >> 1;
-    const int nstrips = (W + CORE - 1) / CORE, nseg = (H + S - 1) / S;
-    const bool first = lev == nlev - 1;
-    for (int t = wave; t < nstrips * nseg; t += nwaves) {
-      const int strip = t / nseg, seg = t - strip * nseg;
-      if (first) {
-        const GlobalLoad<T> llsrc{A, ap};
-        if (lev == 0)
-          inv_tile<T, NS, U>(strip, seg, S, W, H, llsrc, A, ap, GlobalOut<T, false>{dst, dstp}, cf);
-        else
-          inv_tile<T, NS, U>(strip, seg, S, W, H, llsrc, A, ap,
-                             LdsOut<T>{lds + (lev & 1) * buf1_off, W}, cf);
-      } else {
-        const LdsLoad<T> llsrc{lds + ((lev + 1) & 1) * buf1_off, Wc};
-        if (lev == 0)
-          inv_tile<T, NS, U>(strip, seg, S, W, H, llsrc, A, ap, GlobalOut<T, false>{dst, dstp}, cf);
-        else
-          inv_tile<T, NS, U>(strip, seg, S, W, H, llsrc, A, ap,
-                             LdsOut<T>{lds + (lev & 1) * buf1_off, W}, cf);
-      }
-    }
-    __syncthreads();
+    lds_permute(b0, b1, W0, W, H, true, false);      // rows
+    lds_scale_pass(b1, W0, W, H, true, cf.se, cf.so);
+    lds_lift_pass(b1, W0, H, W, true, cf);
+    lds_permute(b1, b0, W0, W, H, false, false);     // columns
+    lds_scale_pass(b0, W0, W, H, false, cf.se, cf.so);
+    lds_lift_pass(b0, W0, W, H, false, cf);
+  }
+  for (int w = threadIdx.x; w < W0 * H0; w += blockDim.x) {
+    const int y = w / W0, x = w - y * W0;
+    dst[(long long)y * dp + x] = b0[w];
   }
 }
 
@@ -631,49 +684,33 @@ void launch_inv_level(const T* ll, long long llp, const T* det, long long dp, T*
   }
 }
 
-// ---- tail launchers --------------------------------------------------------------
-// A tail starts at level jt >= 1 (level 0 is in place on the caller's image and
-// always runs wide) once the level input is at most g_tail_bytes, provided the
-// two LDS buffers for its LL outputs fit.
-static long long g_tail_bytes = 256 * 1024;
-static int g_tail_seg_rows = 8;
-constexpr int kTailLdsMax = 150 * 1024;
+// ---- block (tail) launchers --------------------------------------------------------
+// The block kernel takes over from the first level whose region fits the LDS
+// budget (two buffers of W x H elements).  Level 0 may be included: the block
+// kernel reads its whole region before writing, so it works in place.
+static long long g_block_bytes = 64 * 1024;  // one LDS buffer
+constexpr int kBlockLdsMax = 150 * 1024;
 
-void tail_config(long long max_input_bytes, int seg_rows) {
-  g_tail_bytes = max_input_bytes;
-  if (seg_rows >= 2 && !(seg_rows & 1)) g_tail_seg_rows = seg_rows;
-}
-
-int tail_lds_bytes(int W0, int H0, int elem) {
-  const long long w1 = ceil_div_pow2(W0, 1), h1 = ceil_div_pow2(H0, 1);
-  const long long w2 = ceil_div_pow2(W0, 2), h2 = ceil_div_pow2(H0, 2);
-  const long long b = (w1 * h1 + w2 * h2) * elem;
-  return b > (1 << 30) ? (1 << 30) : (int)b;
+void tail_config(long long max_region_bytes, int) {
+  g_block_bytes = max_region_bytes > kBlockLdsMax / 2 ? kBlockLdsMax / 2 : max_region_bytes;
 }
 
 int tail_start(int W, int H, int L, int elem) {
-  if (g_tail_bytes <= 0) return L;
-  for (int j = 1; j < L; ++j) {
+  if (g_block_bytes <= 0) return L;
+  for (int j = 0; j < L; ++j) {
     const long long wj = ceil_div_pow2(W, j), hj = ceil_div_pow2(H, j);
-    if (wj * hj * elem <= g_tail_bytes &&
-        tail_lds_bytes((int)wj, (int)hj, elem) <= kTailLdsMax)
-      return j;
+    if (wj * hj * elem <= g_block_bytes) return j;
   }
   return L;
 }
 
 hipError_t configure_tail_kernels() {
-  const void* fns[] = {
-      reinterpret_cast<const void*>(&fwd_tail_kernel<float, 4, 4>),
-      reinterpret_cast<const void*>(&fwd_tail_kernel<float, 2, 4>),
-      reinterpret_cast<const void*>(&fwd_tail_kernel<double, 4, 4>),
-      reinterpret_cast<const void*>(&fwd_tail_kernel<double, 2, 4>),
-      reinterpret_cast<const void*>(&inv_tail_kernel<float, 4, 4>),
-      reinterpret_cast<const void*>(&inv_tail_kernel<float, 2, 4>),
-      reinterpret_cast<const void*>(&inv_tail_kernel<double, 4, 4>),
-      reinterpret_cast<const void*>(&inv_tail_kernel<double, 2, 4>)};
+  const void* fns[] = {reinterpret_cast<const void*>(&block_fwd_kernel<float>),
+                       reinterpret_cast<const void*>(&block_fwd_kernel<double>),
+                       reinterpret_cast<const void*>(&block_inv_kernel<float>),
+                       reinterpret_cast<const void*>(&block_inv_kernel<double>)};
   for (const void* f : fns) {
-    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kTailLdsMax);
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kBlockLdsMax);
     if (e != hipSuccess) {
       (void)hipGetLastError();
       return e;
@@ -683,41 +720,29 @@ hipError_t configure_tail_kernels() {
 }
 
 template <typename T>
-void launch_fwd_tail(const T* src, long long sp, int W0, int H0, int nlev, T* A, long long ap,
-                     const Coeffs<T>& cf, hipStream_t st) {
-  const long long w1 = ceil_div_pow2(W0, 1), h1 = ceil_div_pow2(H0, 1);
-  const int lds = tail_lds_bytes(W0, H0, sizeof(T));
-  const int off = (int)(w1 * h1);
-  if (cf.nsteps == 4)
-    hipLaunchKernelGGL((fwd_tail_kernel<T, 4, 4>), dim3(1), dim3(kTailThreads), lds, st, src, sp,
-                       W0, H0, nlev, A, ap, g_tail_seg_rows, off, cf);
-  else
-    hipLaunchKernelGGL((fwd_tail_kernel<T, 2, 4>), dim3(1), dim3(kTailThreads), lds, st, src, sp,
-                       W0, H0, nlev, A, ap, g_tail_seg_rows, off, cf);
+void launch_block_fwd(const T* src, long long sp, T* dst, long long dp, int W0, int H0, int nlev,
+                      const Coeffs<T>& cf, hipStream_t st) {
+  const size_t lds = 2 * (size_t)W0 * H0 * sizeof(T);
+  hipLaunchKernelGGL((block_fwd_kernel<T>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst, dp,
+                     W0, H0, nlev, cf);
 }
 
 template <typename T>
-void launch_inv_tail(const T* A, long long ap, int W0, int H0, int nlev, T* dst, long long dstp,
-                     const Coeffs<T>& cf, hipStream_t st) {
-  const long long w2 = ceil_div_pow2(W0, 2), h2 = ceil_div_pow2(H0, 2);
-  const int lds = tail_lds_bytes(W0, H0, sizeof(T));
-  const int off = (int)(w2 * h2);
-  if (cf.nsteps == 4)
-    hipLaunchKernelGGL((inv_tail_kernel<T, 4, 4>), dim3(1), dim3(kTailThreads), lds, st, A, ap,
-                       W0, H0, nlev, dst, dstp, g_tail_seg_rows, off, cf);
-  else
-    hipLaunchKernelGGL((inv_tail_kernel<T, 2, 4>), dim3(1), dim3(kTailThreads), lds, st, A, ap,
-                       W0, H0, nlev, dst, dstp, g_tail_seg_rows, off, cf);
+void launch_block_inv(const T* src, long long sp, T* dst, long long dp, int W0, int H0, int nlev,
+                      const Coeffs<T>& cf, hipStream_t st) {
+  const size_t lds = 2 * (size_t)W0 * H0 * sizeof(T);
+  hipLaunchKernelGGL((block_inv_kernel<T>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst, dp,
+                     W0, H0, nlev, cf);
 }
 
-template void launch_fwd_tail<float>(const float*, long long, int, int, int, float*, long long,
-                                     const Coeffs<float>&, hipStream_t);
-template void launch_fwd_tail<double>(const double*, long long, int, int, int, double*, long long,
-                                      const Coeffs<double>&, hipStream_t);
-template void launch_inv_tail<float>(const float*, long long, int, int, int, float*, long long,
-                                     const Coeffs<float>&, hipStream_t);
-template void launch_inv_tail<double>(const double*, long long, int, int, int, double*, long long,
-                                      const Coeffs<double>&, hipStream_t);
+template void launch_block_fwd<float>(const float*, long long, float*, long long, int, int, int,
+                                      const Coeffs<float>&, hipStream_t);
+template void launch_block_fwd<double>(const double*, long long, double*, long long, int, int, int,
+                                       const Coeffs<double>&, hipStream_t);
+template void launch_block_inv<float>(const float*, long long, float*, long long, int, int, int,
+                                      const Coeffs<float>&, hipStream_t);
+template void launch_block_inv<double>(const double*, long long, double*, long long, int, int, int,
+                                       const Coeffs<double>&, hipStream_t);
 
 template <typename T>
 void launch_copy_rect(const T* src, long long sp, T* dst, long long dp, int w, int h,
