@@ -451,68 +451,111 @@ inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ 
 //
 // Within LDS a level is computed line-parallel by the whole workgroup, each
 // lifting step a barrier-separated sweep (x + c * (l + r), a missing neighbour
-// at a line end replaced by the existing one), then the samples are
-// deinterleaved into Mallat order through the second LDS buffer.  Rows first,
-// then columns, in both directions.
+// at a line end replaced by the existing one); the parity scaling is folded
+// into the move between Mallat and interleaved order through the second LDS
+// buffer.  Rows first, then columns, in both directions.
 constexpr int kBlockThreads = 1024;
 
+// Lifting steps on the H rows (length W >= 2) of buf: wave w takes rows
+// w, w + nwaves, ...; lane l takes the samples of the step's parity at
+// positions par + 2l, par + 2l + 128, ...
 template <typename T>
-__device__ __forceinline__ void lds_lift_pass(Lds<T>* buf, int pitch, int nlines, int N,
-                                              bool rows, const Coeffs<T>& cf) {
-  // lifting steps on nlines lines of length N >= 2 held interleaved in buf
-  // (rows: line y is buf[y*pitch + i]; columns: line x is buf[i*pitch + x])
+__device__ __forceinline__ void lds_lift_rows(Lds<T>* buf, int pitch, int W, int H,
+                                              const Coeffs<T>& cf) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int st = 0; st < cf.nsteps; ++st) {
     const int par = (cf.first_parity + st) & 1;
-    const int cnt = (N - par + 1) >> 1;  // samples of this parity
     const T c = cf.c[st];
     const T c2 = T(2) * c;
-    for (int w = threadIdx.x; w < nlines * cnt; w += blockDim.x) {
-      const int line = w / cnt, i = par + 2 * (w - line * cnt);
-      const int base = rows ? line * pitch : line;
-      const int step = rows ? 1 : pitch;
-      Lds<T>* x = buf + base + i * step;
-      T v;
-      if (i > 0 && i + 1 < N) {
-        T t = x[-step] + x[step];
-        t = c * t;
-        v = x[0] + t;
-      } else {
-        T t = c2 * (i > 0 ? x[-step] : x[step]);
-        v = x[0] + t;
+    for (int y = wave; y < H; y += nw) {
+      Lds<T>* row = buf + y * pitch;
+      for (int i = par + 2 * lane; i < W; i += 2 * kWave) {
+        T t;
+        if (i > 0 && i + 1 < W) {
+          t = row[i - 1] + row[i + 1];
+          t = c * t;
+        } else {
+          t = c2 * (i > 0 ? row[i - 1] : row[i + 1]);
+        }
+        row[i] = row[i] + t;
       }
-      x[0] = v;
     }
     __syncthreads();
   }
 }
 
+// Lifting steps on the W columns (length H >= 2) of buf: lane l takes columns
+// l, l + 64, ... (consecutive lanes, consecutive LDS banks); wave w takes the
+// step's positions par + 2w, par + 2w + 2 nwaves, ...
 template <typename T>
-__device__ __forceinline__ void lds_scale_pass(Lds<T>* buf, int pitch, int W, int H, bool rows,
-                                               T se, T so) {
-  for (int w = threadIdx.x; w < W * H; w += blockDim.x) {
-    const int y = w / W, x = w - y * W;
-    const int par = rows ? (x & 1) : (y & 1);
-    buf[y * pitch + x] = buf[y * pitch + x] * (par ? so : se);
+__device__ __forceinline__ void lds_lift_cols(Lds<T>* buf, int pitch, int W, int H,
+                                              const Coeffs<T>& cf) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int st = 0; st < cf.nsteps; ++st) {
+    const int par = (cf.first_parity + st) & 1;
+    const T c = cf.c[st];
+    const T c2 = T(2) * c;
+    for (int x = lane; x < W; x += kWave) {
+      for (int i = par + 2 * wave; i < H; i += 2 * nw) {
+        Lds<T>* e = buf + i * pitch + x;
+        T t;
+        if (i > 0 && i + 1 < H) {
+          t = e[-pitch] + e[pitch];
+          t = c * t;
+        } else {
+          t = c2 * (i > 0 ? e[-pitch] : e[pitch]);
+        }
+        e[0] = e[0] + t;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Permutation with the parity scaling folded in, over the W x H region.
+// Forward (deinterleave): out[.., j] = in[.., i] * (i odd ? so : se) with even
+// i to j = i/2 and odd i to j = ceil(N/2) + i/2.  Inverse (interleave):
+// out[.., j] = in[.., i] * (j odd ? so : se) with i < ceil(N/2) to j = 2i, the
+// rest to j = 2(i - ceil(N/2)) + 1.  Along rows (ROWS) or columns.
+template <typename T, bool ROWS, bool DEINTERLEAVE>
+__device__ __forceinline__ void lds_permute_scale(const Lds<T>* in, Lds<T>* out, int pitch, int W,
+                                                  int H, T se, T so) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int N = ROWS ? W : H, half = (N + 1) >> 1;
+  for (int y = wave; y < H; y += nw) {
+    for (int x = lane; x < W; x += kWave) {
+      const int i = ROWS ? x : y;
+      int j;
+      bool odd;
+      if (DEINTERLEAVE) {
+        j = (i & 1) ? half + (i >> 1) : (i >> 1);
+        odd = i & 1;
+      } else {
+        j = i < half ? 2 * i : 2 * (i - half) + 1;
+        odd = j & 1;
+      }
+      const int yo = ROWS ? y : j, xo = ROWS ? j : x;
+      out[yo * pitch + xo] = in[y * pitch + x] * (odd ? so : se);
+    }
   }
   __syncthreads();
 }
 
-// out = permutation of in over the W x H region: forward deinterleave (even
-// samples to [0, ceil(N/2)), odd to [ceil(N/2), N)) or inverse interleave, along
-// rows or columns.
 template <typename T>
-__device__ __forceinline__ void lds_permute(const Lds<T>* in, Lds<T>* out, int pitch, int W, int H,
-                                            bool rows, bool deinterleave) {
-  const int N = rows ? W : H, half = (N + 1) >> 1;
-  for (int w = threadIdx.x; w < W * H; w += blockDim.x) {
-    const int y = w / W, x = w - y * W;
-    const int i = rows ? x : y;
-    const int j = deinterleave ? ((i & 1) ? half + (i >> 1) : (i >> 1))
-                               : (i < half ? 2 * i : 2 * (i - half) + 1);
-    const int yo = rows ? y : j, xo = rows ? j : x;
-    out[yo * pitch + xo] = in[y * pitch + x];
-  }
+__device__ __forceinline__ void block_load(const T* __restrict__ src, long long sp, Lds<T>* b,
+                                           int W, int H) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int y = wave; y < H; y += nw)
+    for (int x = lane; x < W; x += kWave) b[y * W + x] = src[(long long)y * sp + x];
   __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void block_store(const Lds<T>* b, T* __restrict__ dst, long long dp,
+                                            int W, int H) {
+  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int y = wave; y < H; y += nw)
+    for (int x = lane; x < W; x += kWave) dst[(long long)y * dp + x] = b[y * W + x];
 }
 
 // Forward: levels 0..nlev-1 of the W0 x H0 region read from src (pitch sp),
@@ -524,24 +567,15 @@ block_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, l
   extern __shared__ __align__(16) unsigned char blk_smem[];
   Lds<T>* const b0 = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
   Lds<T>* const b1 = b0 + W0 * H0;
-  for (int w = threadIdx.x; w < W0 * H0; w += blockDim.x) {
-    const int y = w / W0, x = w - y * W0;
-    b0[w] = src[(long long)y * sp + x];
-  }
-  __syncthreads();
+  block_load(src, sp, b0, W0, H0);
   for (int lev = 0; lev < nlev; ++lev) {
     const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
-    lds_lift_pass(b0, W0, H, W, true, cf);           // rows
-    lds_scale_pass(b0, W0, W, H, true, cf.se, cf.so);
-    lds_permute(b0, b1, W0, W, H, true, true);
-    lds_lift_pass(b1, W0, W, H, false, cf);          // columns
-    lds_scale_pass(b1, W0, W, H, false, cf.se, cf.so);
-    lds_permute(b1, b0, W0, W, H, false, true);
+    lds_lift_rows(b0, W0, W, H, cf);  // rows: lift, then scale into Mallat order
+    lds_permute_scale<T, true, true>(b0, b1, W0, W, H, cf.se, cf.so);
+    lds_lift_cols(b1, W0, W, H, cf);  // columns
+    lds_permute_scale<T, false, true>(b1, b0, W0, W, H, cf.se, cf.so);
   }
-  for (int w = threadIdx.x; w < W0 * H0; w += blockDim.x) {
-    const int y = w / W0, x = w - y * W0;
-    dst[(long long)y * dp + x] = b0[w];
-  }
+  block_store(b0, dst, dp, W0, H0);
 }
 
 // Inverse: the nlev levels whose finest output is W0 x H0; coefficients of the
@@ -553,24 +587,15 @@ block_inv_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, l
   extern __shared__ __align__(16) unsigned char blk_smem[];
   Lds<T>* const b0 = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
   Lds<T>* const b1 = b0 + W0 * H0;
-  for (int w = threadIdx.x; w < W0 * H0; w += blockDim.x) {
-    const int y = w / W0, x = w - y * W0;
-    b0[w] = src[(long long)y * sp + x];
-  }
-  __syncthreads();
+  block_load(src, sp, b0, W0, H0);
   for (int lev = nlev - 1; lev >= 0; --lev) {
     const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
-    lds_permute(b0, b1, W0, W, H, true, false);      // rows
-    lds_scale_pass(b1, W0, W, H, true, cf.se, cf.so);
-    lds_lift_pass(b1, W0, H, W, true, cf);
-    lds_permute(b1, b0, W0, W, H, false, false);     // columns
-    lds_scale_pass(b0, W0, W, H, false, cf.se, cf.so);
-    lds_lift_pass(b0, W0, W, H, false, cf);
+    lds_permute_scale<T, true, false>(b0, b1, W0, W, H, cf.se, cf.so);  // rows: order, scale,
+    lds_lift_rows(b1, W0, W, H, cf);                                    // lift
+    lds_permute_scale<T, false, false>(b1, b0, W0, W, H, cf.se, cf.so);  // columns
+    lds_lift_cols(b0, W0, W, H, cf);
   }
-  for (int w = threadIdx.x; w < W0 * H0; w += blockDim.x) {
-    const int y = w / W0, x = w - y * W0;
-    dst[(long long)y * dp + x] = b0[w];
-  }
+  block_store(b0, dst, dp, W0, H0);
 }
 
 template <typename T>
