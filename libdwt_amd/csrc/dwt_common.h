@@ -44,6 +44,15 @@ __host__ __device__ inline int mirror(int i, int n) {
   return i < n ? i : p - i;
 }
 
+// mirror() for an index that is usually inside [0, n): the in-range test is
+// one compare, the modulo only runs near (or beyond) an end.
+__host__ __device__ inline int mirror_near(int i, int n) {
+  if (i >= 0 && i < n) return i;
+  if (i < 0 && -i < n) return -i;
+  if (i >= n && i <= 2 * n - 2) return 2 * n - 2 - i;
+  return mirror(i, n);
+}
+
 __host__ __device__ inline int ceil_div_pow2(int i, int j) { return (i + (1 << j) - 1) >> j; }
 
 inline int ceil_log2(int x) {

@@ -191,23 +191,24 @@ struct ColInv {
 //   the mirrored column indices, `interior` says the wave's whole window lies
 //   inside the row.
 // StoreLL(m, k, v): LL coefficient (row m, column k) of the level output.
-template <typename T, int NS, int U, class LoadRow, class StoreLL>
-__device__ __forceinline__ void fwd_tile(int strip, int seg, int S, int W, int H,
-                                         const LoadRow& load_row, const StoreLL& store_ll,
-                                         T* __restrict__ det, long long dp, const Coeffs<T>& cf) {
+template <typename T, int NS, int U, bool INTERIOR, class LoadRow, class StoreLL>
+__device__ __forceinline__ void fwd_tile_impl(int strip, int seg, int S, int W, int H,
+                                              const LoadRow& load_row, const StoreLL& store_ll,
+                                              T* __restrict__ det, long long dp,
+                                              const Coeffs<T>& cf) {
   constexpr int HALO = NS;
   constexpr int CORE = 2 * kWave - 2 * HALO;
   constexpr int LAG = NS / 2;
   const int lane = threadIdx.x & (kWave - 1);
   const int x0 = strip * CORE;
   const int Xe = x0 - HALO + 2 * lane;
-  const int xe = mirror(Xe, W), xo = mirror(Xe + 1, W);
+  const int xe = INTERIOR ? Xe : mirror(Xe, W), xo = INTERIOR ? Xe + 1 : mirror(Xe + 1, W);
   const int Wd = (W + 1) >> 1, Wh = W >> 1, Hd = (H + 1) >> 1, Hh = H >> 1;
   const int k = Xe >> 1;
   const bool core = lane >= HALO / 2 && lane < kWave - HALO / 2;
   const bool st_a = core && k < Wd;
   const bool st_b = core && k < Wh;
-  const bool interior = (x0 - HALO >= 0) && (x0 - HALO + 2 * kWave <= W);
+  constexpr bool interior = INTERIOR;
 
   const int m0 = (seg * S) >> 1;
   const int m1 = min((seg * S + S) >> 1, Hd);
@@ -251,6 +252,23 @@ __device__ __forceinline__ void fwd_tile(int strip, int seg, int S, int W, int H
   }
 }
 
+// The whole 128-column window of a strip either lies inside the rows (plain,
+// vectorisable loads) or touches an end (mirrored columns).  The test is the
+// same for every lane of the wavefront; readfirstlane makes it a scalar branch
+// between two separately compiled bodies.
+template <typename T, int NS, int U, class LoadRow, class StoreLL>
+__device__ __forceinline__ void fwd_tile(int strip, int seg, int S, int W, int H,
+                                         const LoadRow& load_row, const StoreLL& store_ll,
+                                         T* __restrict__ det, long long dp, const Coeffs<T>& cf) {
+  constexpr int HALO = NS;
+  const int x0 = strip * (2 * kWave - 2 * HALO);
+  const int interior = __builtin_amdgcn_readfirstlane((x0 - HALO >= 0) && (x0 - HALO + 2 * kWave <= W));
+  if (interior)
+    fwd_tile_impl<T, NS, U, true>(strip, seg, S, W, H, load_row, store_ll, det, dp, cf);
+  else
+    fwd_tile_impl<T, NS, U, false>(strip, seg, S, W, H, load_row, store_ll, det, dp, cf);
+}
+
 // Inverse tile: output columns of strip `strip`, rows of segment `seg` of the
 // W x H fine region.  LoadLL(m, k): LL coefficient; details come from det at
 // their Mallat positions.  StoreOut(r, Xe, e, o, st_e, st_o): output samples
@@ -280,7 +298,7 @@ __device__ __forceinline__ void inv_tile(int strip, int seg, int S, int W, int H
   T ce[2 * U], co[2 * U];
 
   auto load_row = [&](int Y, T& e, T& o) {
-    const int Yr = mirror(Y, H);
+    const int Yr = mirror_near(Y, H);
     const int m = Yr >> 1;
     if (Yr & 1) {
       const T* r = det + (long long)(Hc + m) * dp;
@@ -330,7 +348,7 @@ struct GlobalRows {  // packed rows in global memory
   __device__ __forceinline__ void operator()(int Y, int Xe, int xe, int xo, bool interior, T& e,
                                              T& o) const {
     using V2 = typename Vec2<T>::type;
-    const T* rp = p + (long long)mirror(Y, H) * pitch;
+    const T* rp = p + (long long)mirror_near(Y, H) * pitch;
     if (VEC && interior) {
       const V2 v = *reinterpret_cast<const V2*>(rp + Xe);
       e = v.x;
@@ -353,7 +371,7 @@ struct LdsRows {  // packed rows in LDS (pitch = W)
   const Lds<T>* p;
   int pitch, H;
   __device__ __forceinline__ void operator()(int Y, int, int xe, int xo, bool, T& e, T& o) const {
-    const Lds<T>* rp = p + mirror(Y, H) * pitch;
+    const Lds<T>* rp = p + mirror_near(Y, H) * pitch;
     e = rp[xe];
     o = rp[xo];
   }
@@ -630,9 +648,12 @@ static int height_class(int H) {
 static TileShape pick_tile(int H, int nstrips, bool fwd) {
   const TileShape t = g_tiles[fwd ? 0 : 1][height_class(H)];
   if (t.S > 0) return t;
-  int S = 64;
-  while (S > 8 && (long long)nstrips * ((H + S - 1) / S) < 2048) S >>= 1;
-  return TileShape{S, S <= 16 ? 8 : 4};
+  // default from autotuning sweeps on MI355X (4096^2 and 3001x2047, both
+  // wavelets, both directions): tall segments only pay off on the largest
+  // levels; U = 4 was best or within noise nearly everywhere
+  (void)nstrips;
+  const int S = H >= 4096 ? 32 : H >= 2048 ? 16 : 8;
+  return TileShape{S, 4};
 }
 
 int tile_set(int fwd, int hclass, int S, int U) {
