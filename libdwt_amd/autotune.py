@@ -1,9 +1,9 @@
 """Tile-shape autotuner for the packed level kernels.
 
 A level kernel's shape is (S, U): a wavefront walks S rows of its 128-column
-strip and keeps U row pairs of loads in flight.  Tall segments amortise the
-halo rows, short ones give a level more wavefronts; U = 8 keeps more bytes in
-flight.  The best shape depends on the level's size, so the tuner times every
+strip and keeps U row pairs of loads in flight (U = 0: it requests every row
+of its segment at once).  Tall segments amortise the halo rows, short ones
+give a level more wavefronts; deeper prefetch keeps more bytes in flight.  The best shape depends on the level's size, so the tuner times every
 level of a workload for each candidate shape (HIP events around each launch,
 recorded by the library on the launch stream), keeps the fastest shape per
 (direction, height class = ceil(log2 H)) and installs it with
@@ -16,7 +16,10 @@ import math
 
 import libdwt_amd
 
-CANDIDATES = [(S, U) for S in (8, 16, 32, 64, 128) for U in (4, 8)]
+# (S, U): U = 4 / 8 rolling prefetch with any S; U = 0 single shot (the whole
+# segment requested at once), S in 8, 16, 32
+CANDIDATES = ([(S, 4) for S in (8, 16, 32, 64)] + [(S, 8) for S in (16, 32)]
+              + [(S, 0) for S in (8, 16, 32)])
 
 
 def height_class(h: int) -> int:

@@ -215,39 +215,51 @@ __device__ __forceinline__ void fwd_tile_impl(int strip, int seg, int S, int W, 
   const int pbeg = m0 - LAG, pend = m1 + LAG - 1;
 
   ColFwd<T, NS> ca, cb;
-  T ce[2 * U], co[2 * U];
-#pragma unroll
-  for (int u = 0; u < 2 * U; ++u) load_row(2 * pbeg + u, Xe, xe, xo, interior, ce[u], co[u]);
-
-  for (int pb = pbeg; pb <= pend; pb += U) {
-    T ne[2 * U], no[2 * U];
-#pragma unroll
-    for (int u = 0; u < 2 * U; ++u)
-      load_row(2 * (pb + U) + u, Xe, xe, xo, interior, ne[u], no[u]);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      T e0 = ce[2 * u], o0 = co[2 * u], e1 = ce[2 * u + 1], o1 = co[2 * u + 1];
-      row_fwd<T, NS>(e0, o0, cf);
-      row_fwd<T, NS>(e1, o1, cf);
-      T La, Ha, Lb, Hb;
-      ca.push(e0, e1, cf, La, Ha);
-      cb.push(o0, o1, cf, Lb, Hb);
-      const int m = pb + u - LAG;
-      if (m >= m0 && m < m1) {
-        if (st_a) {
-          store_ll(m, k, La);
-          if (m < Hh) det[(long long)(Hd + m) * dp + k] = Ha;
-        }
-        if (st_b) {
-          det[(long long)m * dp + Wd + k] = Lb;
-          if (m < Hh) det[(long long)(Hd + m) * dp + Wd + k] = Hb;
-        }
+  auto pair = [&](int p, T e0, T o0, T e1, T o1) {
+    row_fwd<T, NS>(e0, o0, cf);
+    row_fwd<T, NS>(e1, o1, cf);
+    T La, Ha, Lb, Hb;
+    ca.push(e0, e1, cf, La, Ha);
+    cb.push(o0, o1, cf, Lb, Hb);
+    const int m = p - LAG;
+    if (m >= m0 && m < m1) {
+      if (st_a) {
+        store_ll(m, k, La);
+        if (m < Hh) det[(long long)(Hd + m) * dp + k] = Ha;
+      }
+      if (st_b) {
+        det[(long long)m * dp + Wd + k] = Lb;
+        if (m < Hh) det[(long long)(Hd + m) * dp + Wd + k] = Hb;
       }
     }
+  };
+
+  if constexpr (U < 0) {
+    // single shot: every row of the segment (S = -U rows plus the halo) is
+    // requested before any is used, one memory round trip per wavefront
+    constexpr int NP = -U / 2 + NS;
+    T re[2 * NP], ro[2 * NP];
 #pragma unroll
-    for (int u = 0; u < 2 * U; ++u) {
-      ce[u] = ne[u];
-      co[u] = no[u];
+    for (int u = 0; u < 2 * NP; ++u) load_row(2 * pbeg + u, Xe, xe, xo, interior, re[u], ro[u]);
+#pragma unroll
+    for (int u = 0; u < NP; ++u) pair(pbeg + u, re[2 * u], ro[2 * u], re[2 * u + 1], ro[2 * u + 1]);
+  } else {
+    // rolling: U row pairs in flight while the previous U are computed
+    T ce[2 * U], co[2 * U];
+#pragma unroll
+    for (int u = 0; u < 2 * U; ++u) load_row(2 * pbeg + u, Xe, xe, xo, interior, ce[u], co[u]);
+    for (int pb = pbeg; pb <= pend; pb += U) {
+      T ne[2 * U], no[2 * U];
+#pragma unroll
+      for (int u = 0; u < 2 * U; ++u)
+        load_row(2 * (pb + U) + u, Xe, xe, xo, interior, ne[u], no[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) pair(pb + u, ce[2 * u], co[2 * u], ce[2 * u + 1], co[2 * u + 1]);
+#pragma unroll
+      for (int u = 0; u < 2 * U; ++u) {
+        ce[u] = ne[u];
+        co[u] = no[u];
+      }
     }
   }
 }
@@ -294,9 +306,6 @@ __device__ __forceinline__ void inv_tile(int strip, int seg, int S, int W, int H
   const int y1 = min(y0 + S, H);
   const int pbeg = (y0 >> 1) - LAG, pend = (y1 >> 1) + LAG + 1;
 
-  ColInv<T, NS> cle, clo;
-  T ce[2 * U], co[2 * U];
-
   auto load_row = [&](int Y, T& e, T& o) {
     const int Yr = mirror_near(Y, H);
     const int m = Yr >> 1;
@@ -310,31 +319,42 @@ __device__ __forceinline__ void inv_tile(int strip, int seg, int S, int W, int H
     }
   };
 
-#pragma unroll
-  for (int u = 0; u < 2 * U; ++u) load_row(2 * pbeg + u, ce[u], co[u]);
-
-  for (int pb = pbeg; pb <= pend; pb += U) {
-    T ne[2 * U], no[2 * U];
-#pragma unroll
-    for (int u = 0; u < 2 * U; ++u) load_row(2 * (pb + U) + u, ne[u], no[u]);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      T e0 = ce[2 * u], o0 = co[2 * u], e1 = ce[2 * u + 1], o1 = co[2 * u + 1];
-      row_inv<T, NS>(e0, o0, cf);
-      row_inv<T, NS>(e1, o1, cf);
-      T fe, se_, fo, so_;
-      cle.push(e0, e1, cf, fe, se_);
-      clo.push(o0, o1, cf, fo, so_);
-      const int r0 = NS == 4 ? 2 * (pb + u) - 3 : 2 * (pb + u) - 2;
-      if (core) {
-        if (r0 >= y0 && r0 < y1) store_out(r0, Xe, fe, fo, st_e, st_o);
-        if (r0 + 1 >= y0 && r0 + 1 < y1) store_out(r0 + 1, Xe, se_, so_, st_e, st_o);
-      }
+  ColInv<T, NS> cle, clo;
+  auto pair = [&](int p, T e0, T o0, T e1, T o1) {
+    row_inv<T, NS>(e0, o0, cf);
+    row_inv<T, NS>(e1, o1, cf);
+    T fe, se_, fo, so_;
+    cle.push(e0, e1, cf, fe, se_);
+    clo.push(o0, o1, cf, fo, so_);
+    const int r0 = NS == 4 ? 2 * p - 3 : 2 * p - 2;
+    if (core) {
+      if (r0 >= y0 && r0 < y1) store_out(r0, Xe, fe, fo, st_e, st_o);
+      if (r0 + 1 >= y0 && r0 + 1 < y1) store_out(r0 + 1, Xe, se_, so_, st_e, st_o);
     }
+  };
+
+  if constexpr (U < 0) {
+    constexpr int NP = -U / 2 + NS;  // pairs y0/2 - LAG .. y0/2 + S/2 + LAG - 1
+    T re[2 * NP], ro[2 * NP];
 #pragma unroll
-    for (int u = 0; u < 2 * U; ++u) {
-      ce[u] = ne[u];
-      co[u] = no[u];
+    for (int u = 0; u < 2 * NP; ++u) load_row(2 * pbeg + u, re[u], ro[u]);
+#pragma unroll
+    for (int u = 0; u < NP; ++u) pair(pbeg + u, re[2 * u], ro[2 * u], re[2 * u + 1], ro[2 * u + 1]);
+  } else {
+    T ce[2 * U], co[2 * U];
+#pragma unroll
+    for (int u = 0; u < 2 * U; ++u) load_row(2 * pbeg + u, ce[u], co[u]);
+    for (int pb = pbeg; pb <= pend; pb += U) {
+      T ne[2 * U], no[2 * U];
+#pragma unroll
+      for (int u = 0; u < 2 * U; ++u) load_row(2 * (pb + U) + u, ne[u], no[u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) pair(pb + u, ce[2 * u], co[2 * u], ce[2 * u + 1], co[2 * u + 1]);
+#pragma unroll
+      for (int u = 0; u < 2 * U; ++u) {
+        ce[u] = ne[u];
+        co[u] = no[u];
+      }
     }
   }
 }
@@ -659,7 +679,9 @@ static TileShape pick_tile(int H, int nstrips, bool fwd) {
 int tile_set(int fwd, int hclass, int S, int U) {
   if (hclass < 0 || hclass >= 32) return -1;
   if (S != 0 && (S < 2 || (S & 1) || S > 4096)) return -1;
-  if (S != 0 && U != 4 && U != 8) return -1;
+  // U = 4 or 8: rolling prefetch of U row pairs; U = 0: single shot, the
+  // whole segment requested at once (S = 8, 16 or 32)
+  if (S != 0 && U != 4 && U != 8 && !(U == 0 && (S == 8 || S == 16 || S == 32))) return -1;
   g_tiles[fwd ? 0 : 1][hclass] = TileShape{S, U};
   return 0;
 }
@@ -677,12 +699,53 @@ static bool aligned_pair(const void* p, long long pitch) {
   return (reinterpret_cast<uintptr_t>(p) % (2 * sizeof(T))) == 0 && (pitch % 2) == 0;
 }
 
-#define DWT_DISPATCH_FWD(NS, VEC, U)                                                        \
-  hipLaunchKernelGGL((fwd_level_kernel<T, NS, VEC, U>), grid, block, 0, st, src, sp, W, H, ll, \
-                     llp, det, dp, t.S, nstrips, nseg, cf)
-#define DWT_DISPATCH_INV(NS, VEC, U)                                                           \
-  hipLaunchKernelGGL((inv_level_kernel<T, NS, VEC, U>), grid, block, 0, st, ll, llp, det, dp, dst, \
-                     dstp, W, H, t.S, nstrips, nseg, cf)
+// Instantiation for a tile shape: rolling prefetch (U = 4 / 8, any S) or
+// single shot (U = 0, S in {8, 16, 32}, template U = -S).
+template <typename T, int NS, bool VEC, class Launch>
+static void with_shape(const TileShape& t, const Launch& launch) {
+  if (t.U == 0) {
+    if (t.S == 8) launch.template go<T, NS, VEC, -8>();
+    else if (t.S == 16) launch.template go<T, NS, VEC, -16>();
+    else launch.template go<T, NS, VEC, -32>();
+  } else if (t.U == 8) {
+    launch.template go<T, NS, VEC, 8>();
+  } else {
+    launch.template go<T, NS, VEC, 4>();
+  }
+}
+
+template <typename T, class Launch>
+static void with_variant(int nsteps, bool vec, const TileShape& t, const Launch& launch) {
+  if (nsteps == 4) {
+    if (vec) with_shape<T, 4, true>(t, launch);
+    else with_shape<T, 4, false>(t, launch);
+  } else {
+    if (vec) with_shape<T, 2, true>(t, launch);
+    else with_shape<T, 2, false>(t, launch);
+  }
+}
+
+template <typename T>
+struct FwdLaunch {
+  const T* src; long long sp; int W, H; T* ll; long long llp; T* det; long long dp;
+  int S, nstrips, nseg; dim3 grid, block; Coeffs<T> cf; hipStream_t st;
+  template <typename TT, int NS, bool VEC, int U>
+  void go() const {
+    hipLaunchKernelGGL((fwd_level_kernel<TT, NS, VEC, U>), grid, block, 0, st, src, sp, W, H, ll,
+                       llp, det, dp, S, nstrips, nseg, cf);
+  }
+};
+
+template <typename T>
+struct InvLaunch {
+  const T* ll; long long llp; const T* det; long long dp; T* dst; long long dstp; int W, H;
+  int S, nstrips, nseg; dim3 grid, block; Coeffs<T> cf; hipStream_t st;
+  template <typename TT, int NS, bool VEC, int U>
+  void go() const {
+    hipLaunchKernelGGL((inv_level_kernel<TT, NS, VEC, U>), grid, block, 0, st, ll, llp, det, dp,
+                       dst, dstp, W, H, S, nstrips, nseg, cf);
+  }
+};
 
 template <typename T>
 void launch_fwd_level(const T* src, long long sp, int W, int H, T* ll, long long llp, T* det,
@@ -692,19 +755,9 @@ void launch_fwd_level(const T* src, long long sp, int W, int H, T* ll, long long
   const TileShape t = pick_tile(H, nstrips, true);
   const int nseg = (H + t.S - 1) / t.S;
   const int ngroups = (nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
-  const dim3 grid(ngroups * nseg), block(kWave * kWavesPerBlock);
-  const bool vec = aligned_pair<T>(src, sp);
-  const int sel = (cf.nsteps == 4 ? 4 : 0) + (vec ? 2 : 0) + (t.U == 8 ? 1 : 0);
-  switch (sel) {
-    case 0: DWT_DISPATCH_FWD(2, false, 4); break;
-    case 1: DWT_DISPATCH_FWD(2, false, 8); break;
-    case 2: DWT_DISPATCH_FWD(2, true, 4); break;
-    case 3: DWT_DISPATCH_FWD(2, true, 8); break;
-    case 4: DWT_DISPATCH_FWD(4, false, 4); break;
-    case 5: DWT_DISPATCH_FWD(4, false, 8); break;
-    case 6: DWT_DISPATCH_FWD(4, true, 4); break;
-    default: DWT_DISPATCH_FWD(4, true, 8); break;
-  }
+  const FwdLaunch<T> L{src, sp, W, H, ll, llp, det, dp, t.S, nstrips, nseg,
+                       dim3(ngroups * nseg), dim3(kWave * kWavesPerBlock), cf, st};
+  with_variant<T>(cf.nsteps, aligned_pair<T>(src, sp), t, L);
 }
 
 template <typename T>
@@ -715,26 +768,16 @@ void launch_inv_level(const T* ll, long long llp, const T* det, long long dp, T*
   const TileShape t = pick_tile(H, nstrips, false);
   const int nseg = (H + t.S - 1) / t.S;
   const int ngroups = (nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
-  const dim3 grid(ngroups * nseg), block(kWave * kWavesPerBlock);
-  const bool vec = aligned_pair<T>(dst, dstp);
-  const int sel = (cf.nsteps == 4 ? 4 : 0) + (vec ? 2 : 0) + (t.U == 8 ? 1 : 0);
-  switch (sel) {
-    case 0: DWT_DISPATCH_INV(2, false, 4); break;
-    case 1: DWT_DISPATCH_INV(2, false, 8); break;
-    case 2: DWT_DISPATCH_INV(2, true, 4); break;
-    case 3: DWT_DISPATCH_INV(2, true, 8); break;
-    case 4: DWT_DISPATCH_INV(4, false, 4); break;
-    case 5: DWT_DISPATCH_INV(4, false, 8); break;
-    case 6: DWT_DISPATCH_INV(4, true, 4); break;
-    default: DWT_DISPATCH_INV(4, true, 8); break;
-  }
+  const InvLaunch<T> L{ll, llp, det, dp, dst, dstp, W, H, t.S, nstrips, nseg,
+                       dim3(ngroups * nseg), dim3(kWave * kWavesPerBlock), cf, st};
+  with_variant<T>(cf.nsteps, aligned_pair<T>(dst, dstp), t, L);
 }
 
 // ---- block (tail) launchers --------------------------------------------------------
 // The block kernel takes over from the first level whose region fits the LDS
 // budget (two buffers of W x H elements).  Level 0 may be included: the block
 // kernel reads its whole region before writing, so it works in place.
-static long long g_block_bytes = 64 * 1024;  // one LDS buffer
+static long long g_block_bytes = 16 * 1024;  // one LDS buffer
 constexpr int kBlockLdsMax = 150 * 1024;
 
 void tail_config(long long max_region_bytes, int) {

@@ -101,6 +101,33 @@ def test_config_images(gold, cfg, wavelet):
     assert float(np.abs(rec_img - img).max()) <= FP32_TOL
 
 
+def test_every_tile_shape_is_bit_exact(gold):
+    """Every level-kernel instantiation the tile table can select (rolling
+    prefetch and single shot, all segment heights), on the non-power-of-two
+    benchmark image, both wavelets."""
+    from libdwt_amd import autotune
+    cfg = cases.CONFIG_IMAGES[1]
+    img = workloads.sinusoid_image(cfg["W"], cfg["H"], cfg["seed"])
+    rec = gold["config"][cfg["name"]]
+    lib = libdwt_amd.lib()
+    try:
+        for (S, U) in autotune.CANDIDATES:
+            say(f"tile shape S={S} U={U}")
+            for fwd in (0, 1):
+                for cls in range(32):
+                    assert lib.dwt_gpu_tile_set(fwd, cls, S, U) == 0
+            for w in cases.WAVELETS:
+                t = torch.from_numpy(img).cuda()
+                j = libdwt_amd.fwd2(t, w, cfg["J"])
+                torch.cuda.synchronize()
+                assert cases.digest(t.cpu().numpy()) == rec[w]["fwd"], (S, U, w)
+                libdwt_amd.inv2(t, w, j)
+                torch.cuda.synchronize()
+                assert cases.digest(t.cpu().numpy()) == rec[w]["inv"], (S, U, w)
+    finally:
+        autotune.reset()
+
+
 def test_fp64_config_image_matches_oracle():
     from oracle.oracle import oracle
     say("fp64 512x512 image")
