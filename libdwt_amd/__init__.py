@@ -13,7 +13,7 @@ import os
 import re
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdwt_gpu.so")
+LIB_PATH = os.environ.get("DWT_GPU_LIB") or os.path.join(HERE, "libdwt_gpu.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "libdwt.h")
 
 CDF97, CDF53 = 0, 1

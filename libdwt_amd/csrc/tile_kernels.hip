@@ -65,6 +65,24 @@ __device__ inline double from_prev_lane(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Output stores.  DWT_STORE_POLICY selects the cache policy of every store
+// of a level's results: 0 plain (write-back L2), 1 non-temporal, 2 agent
+// scope (written through to memory, coherent across XCDs, so the end of the
+// kernel has no dirty L2 lines to write back before the next level starts).
+#ifndef DWT_STORE_POLICY
+#define DWT_STORE_POLICY 0
+#endif
+template <typename T>
+__device__ __forceinline__ void st_global(T* p, T v) {
+#if DWT_STORE_POLICY == 1
+  __builtin_nontemporal_store(v, p);
+#elif DWT_STORE_POLICY == 2
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#else
+  *p = v;
+#endif
+}
+
 template <typename T> struct Vec2;
 template <> struct Vec2<float> { using type = float2; };
 template <> struct Vec2<double> { using type = double2; };
@@ -225,11 +243,11 @@ __device__ __forceinline__ void fwd_tile_impl(int strip, int seg, int S, int W, 
     if (m >= m0 && m < m1) {
       if (st_a) {
         store_ll(m, k, La);
-        if (m < Hh) det[(long long)(Hd + m) * dp + k] = Ha;
+        if (m < Hh) st_global(det + (long long)(Hd + m) * dp + k, Ha);
       }
       if (st_b) {
-        det[(long long)m * dp + Wd + k] = Lb;
-        if (m < Hh) det[(long long)(Hd + m) * dp + Wd + k] = Hb;
+        st_global(det + (long long)m * dp + Wd + k, Lb);
+        if (m < Hh) st_global(det + (long long)(Hd + m) * dp + Wd + k, Hb);
       }
     }
   };
@@ -400,7 +418,9 @@ template <typename T>
 struct GlobalStore {  // element store, global
   T* __restrict__ p;
   long long pitch;
-  __device__ __forceinline__ void operator()(int m, int k, T v) const { p[(long long)m * pitch + k] = v; }
+  __device__ __forceinline__ void operator()(int m, int k, T v) const {
+    st_global(p + (long long)m * pitch + k, v);
+  }
 };
 template <typename T>
 struct LdsStore {
@@ -427,15 +447,17 @@ struct GlobalOut {  // two adjacent output samples, global
   __device__ __forceinline__ void operator()(int r, int Xe, T e, T o, bool st_e, bool st_o) const {
     using V2 = typename Vec2<T>::type;
     T* rp = p + (long long)r * pitch + Xe;
+#if DWT_STORE_POLICY == 0
     if (VEC && st_o) {
       V2 v;
       v.x = e;
       v.y = o;
       *reinterpret_cast<V2*>(rp) = v;
-    } else {
-      if (st_e) rp[0] = e;
-      if (st_o) rp[1] = o;
+      return;
     }
+#endif
+    if (st_e) st_global(rp, e);
+    if (st_o) st_global(rp + 1, o);
   }
 };
 template <typename T>
@@ -593,7 +615,7 @@ __device__ __forceinline__ void block_store(const Lds<T>* b, T* __restrict__ dst
                                             int W, int H) {
   const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int y = wave; y < H; y += nw)
-    for (int x = lane; x < W; x += kWave) dst[(long long)y * dp + x] = b[y * W + x];
+    for (int x = lane; x < W; x += kWave) st_global(dst + (long long)y * dp + x, (T)b[y * W + x]);
 }
 
 // Forward: levels 0..nlev-1 of the W0 x H0 region read from src (pitch sp),
@@ -643,7 +665,7 @@ __global__ void copy_rect_kernel(const T* __restrict__ src, long long sp, T* __r
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
     const int y = (int)(t / w), x = (int)(t - (long long)y * w);
-    dst[y * dp + x] = src[y * sp + x];
+    st_global(dst + y * dp + x, src[y * sp + x]);
   }
 }
 
@@ -669,11 +691,11 @@ static TileShape pick_tile(int H, int nstrips, bool fwd) {
   const TileShape t = g_tiles[fwd ? 0 : 1][height_class(H)];
   if (t.S > 0) return t;
   // default from autotuning sweeps on MI355X (4096^2 and 3001x2047, both
-  // wavelets, both directions): tall segments only pay off on the largest
-  // levels; U = 4 was best or within noise nearly everywhere
+  // wavelets, both directions): 8-row segments loaded in a single shot were
+  // fastest or within noise on every level size
   (void)nstrips;
-  const int S = H >= 4096 ? 32 : H >= 2048 ? 16 : 8;
-  return TileShape{S, 4};
+  (void)H;
+  return TileShape{8, 0};
 }
 
 int tile_set(int fwd, int hclass, int S, int U) {
