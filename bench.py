@@ -26,6 +26,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fwd_level_4096x4096_cdf97.md")
+
+
+def measured_traffic(width: int, height: int, wavelet: str):
+    """HBM bytes per launch of the level-0 forward kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy);
+    None when no summary exists for this workload."""
+    if (width, height, wavelet) != (4096, 4096, "cdf97") or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1].split()[0])
+    return None
 
 
 def parse():
@@ -213,7 +227,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None,
+            "traffic": measured_traffic(W, H, wv),
+            "algorithmic_bytes": alg_bytes,
+            "avg_launch_us": lv0["avg_us"] if lv0 else None,
         },
         "cold": ({"ms_per_step_median": round(cold[len(cold) // 2], 5),
                   "value": round(W * H / (cold[len(cold) // 2] * 1e-3) / 1e6, 3),

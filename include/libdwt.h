@@ -153,7 +153,7 @@ int dwt_gpu_tile_get(int forward, int width, int height, int wavelet, int *seg_r
 
 /* The small levels of a transform run as one single-workgroup launch that
  * keeps the whole small pyramid in LDS, from the first level whose region is
- * at most max_region_bytes (default 64 KiB, at most 75 KiB; 0 disables it).
+ * at most max_region_bytes (default 16 KiB, at most 75 KiB; 0 disables it).
  * An image that small is transformed completely by that one launch. */
 int dwt_gpu_tail_set(long long max_region_bytes);
 
