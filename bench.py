@@ -54,6 +54,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--schedule", type=int, default=0, choices=(0, 1),
+                    help="levels after level 0: 1 one pyramid launch, 0 one launch per level")
     ap.add_argument("--cold-steps", type=int, default=10,
                     help="steps timed one at a time after a 512 MiB cache scrub")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -128,6 +130,7 @@ def main():
         return
     lib = libdwt_amd.lib()
     assert lib.dwt_gpu_init() == 0, lib.dwt_gpu_last_error()
+    libdwt_amd.schedule_set(args.schedule)
 
     W, H, J, wv = args.width, args.height, args.levels, args.wavelet
     img = workloads.sinusoid_image(W, H, seed=3)
@@ -157,6 +160,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_events = ev0.elapsed_time(ev1)
+    libdwt_amd.device_check(sp)  # raises if a persistent launch hit its wait limit
     ms_per_step = ms_events / args.steps
 
     # correctness of what was timed: fwd+inv round trip error after K steps
@@ -190,7 +194,7 @@ def main():
     libdwt_amd.profile_enable(False)
     kernels = {}
     for kind, name in ((0, "fwd_level"), (1, "inv_level"), (2, "detail_copy"), (4, "fwd_block"),
-                       (5, "inv_block")):
+                       (5, "inv_block"), (6, "fwd_pyramid"), (7, "inv_pyramid")):
         for lev in range(J):
             n, ms = libdwt_amd.profile_read(kind, lev)
             if n:
@@ -234,6 +238,7 @@ def main():
         "cold": ({"ms_per_step_median": round(cold[len(cold) // 2], 5),
                   "value": round(W * H / (cold[len(cold) // 2] * 1e-3) / 1e6, 3),
                   "steps": len(cold), "scrub_mib": 512} if cold else None),
+        "schedule": "pyramid" if args.schedule == 1 else "per-level",
         "kernels": kernels,
         "wall_s": round(wall, 4),
         "max_abs_roundtrip_err": rec_err,

@@ -135,8 +135,9 @@ int dwt_gpu_2d_async(int wavelet, int direction, int precision, void *ptr, int s
 /* Per-launch timing with HIP events recorded on the launch stream around each
  * kernel of the packed path.  kind: 0 forward level kernel, 1 inverse level
  * kernel, 2 level-0 detail copy, 4 forward block kernel, 5 inverse block
- * kernel; level: the level index, the first level it does for a block kernel
- * (-1 = all).
+ * kernel, 6 forward pyramid, 7 inverse pyramid (levels after level 0);
+ * level: the level index, the first level it does for a block kernel or a
+ * pyramid (-1 = all).
  * profile_read synchronises on the recorded events. */
 int dwt_gpu_profile_enable(int on);
 int dwt_gpu_profile_read(int kind, int level, long long *launches, double *total_ms);
@@ -156,6 +157,18 @@ int dwt_gpu_tile_get(int forward, int width, int height, int wavelet, int *seg_r
  * at most max_region_bytes (default 16 KiB, at most 75 KiB; 0 disables it).
  * An image that small is transformed completely by that one launch. */
 int dwt_gpu_tail_set(long long max_region_bytes);
+
+/* Schedule of the levels after level 0 on the packed path: 0 (default) runs
+ * one launch per level plus a single-workgroup launch for the small levels;
+ * 1 runs them as one persistent launch in which each level's work items start
+ * as soon as the rows they read are finished (dataflow, no barrier between
+ * levels). */
+int dwt_gpu_schedule_set(int mode);
+/* Synchronise `stream` and report whether a persistent launch of this thread
+ * on it hit its wait time limit since the last check (DWT_GPU_EHIP; the
+ * results of that launch are then invalid).  The drop-in calls check this
+ * themselves before returning. */
+int dwt_gpu_device_check(void *stream);
 
 /* Bytes of device scratch the calling thread currently holds for `stream`. */
 long long dwt_gpu_scratch_bytes(void *stream);

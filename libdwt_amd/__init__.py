@@ -89,6 +89,10 @@ def lib() -> C.CDLL:
         L.dwt_gpu_scratch_bytes.restype = C.c_longlong
         L.dwt_gpu_release_scratch.argtypes = [vp]
         L.dwt_gpu_release_scratch.restype = i
+        L.dwt_gpu_schedule_set.argtypes = [i]
+        L.dwt_gpu_schedule_set.restype = i
+        L.dwt_gpu_device_check.argtypes = [vp]
+        L.dwt_gpu_device_check.restype = i
         _lib = L
     return _lib
 
@@ -138,6 +142,17 @@ def transform_2d_async(wavelet: str, forward: bool, ptr: int, stride_x: int, str
     return jj.value
 
 
+def schedule_set(mode: int) -> None:
+    """1: levels after level 0 as one pyramid (dataflow) launch; 0: one launch
+    per level."""
+    _check(lib().dwt_gpu_schedule_set(mode))
+
+
+def device_check(stream: int = 0) -> None:
+    """Raise if a pyramid launch on `stream` (this thread) hit its wait limit."""
+    _check(lib().dwt_gpu_device_check(stream))
+
+
 def profile_enable(on: bool) -> None:
     lib().dwt_gpu_profile_enable(1 if on else 0)
 
@@ -149,7 +164,8 @@ def profile_reset() -> None:
 def profile_read(kind: int, level: int = -1) -> tuple[int, float]:
     """(launches, total milliseconds) of the recorded launches of one kind
     (0 forward level, 1 inverse level, 2 detail copy, 4 forward block kernel,
-    5 inverse block kernel) at one level (-1 all)."""
+    5 inverse block kernel, 6 forward pyramid, 7 inverse pyramid) at one level
+    (-1 all)."""
     n = C.c_longlong(0)
     ms = C.c_double(0)
     _check(lib().dwt_gpu_profile_read(kind, level, C.byref(n), C.byref(ms)))

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -143,10 +144,30 @@ struct DevBuf {
   size_t n = 0;
 };
 
+// A pyramid launch's task table and dependency ranges, uploaded once and
+// reused by every later call with the same tasks (same sizes and pointers).
+// The pinned host copy the upload reads is kept unchanged for the plan's life,
+// so the upload is stream-ordered and safe to replay.
+struct PyrPlan {
+  std::vector<PyrTask> tasks;
+  bool fwd = true, vec = true;
+  int S = 8, nsteps = 4, nitems = 0, ncnt = 0;
+  size_t deps_off = 0;
+  void* dev = nullptr;
+  void* pinned = nullptr;
+};
+
 struct Scratch {
   // 0: LL of even levels, 1: LL of odd levels, 2: level-0 details,
-  // 3: line temp, 4: host staging
-  DevBuf buf[5];
+  // 3: line temp, 4: host staging, 5: LL of the pyramid levels
+  DevBuf buf[6];
+  // pyramid state: plans (most recent first), and the sync words
+  // ctrl[0..1], err[2], then per-segment counters
+  std::vector<PyrPlan*> plans;
+  unsigned* sync = nullptr;
+  long long sync_cnt = 0;
+  unsigned* err_host = nullptr;
+  bool pyr_used = false;
   hipStream_t stream = nullptr;
   // side stream for the level-0 detail copies, which touch memory disjoint
   // from what the other levels read and write, so they overlap those levels
@@ -181,13 +202,27 @@ struct Scratch {
   long long bytes() const {
     long long s = 0;
     for (const auto& b : buf) s += (long long)b.n;
-    return s;
+    for (const PyrPlan* pl : plans)
+      s += (long long)(pl->deps_off + sizeof(int2) * (size_t)pl->ncnt);
+    return s + 4 * (4 + sync_cnt);
   }
   void release() {
     for (auto& b : buf) {
       if (b.p) hipFree(b.p);
       b = DevBuf{};
     }
+    for (PyrPlan* pl : plans) free_plan(pl);
+    plans.clear();
+    if (sync) hipFree(sync);
+    sync = nullptr;
+    sync_cnt = 0;
+    if (err_host) hipHostFree(err_host);
+    err_host = nullptr;
+  }
+  static void free_plan(PyrPlan* pl) {
+    if (pl->dev) hipFree(pl->dev);
+    if (pl->pinned) hipHostFree(pl->pinned);
+    delete pl;
   }
 };
 
@@ -210,7 +245,8 @@ static Scratch& scratch_for(hipStream_t st) {
 
 // ---- optional per-launch timing (HIP events on the launch stream) ---------------
 // Kinds: 0 forward level kernel, 1 inverse level kernel, 2 detail copy,
-// 4 forward block kernel, 5 inverse block kernel.  Level is the level index the
+// 4 forward block kernel, 5 inverse block kernel, 6 forward pyramid,
+// 7 inverse pyramid.  Level is the level index the
 // launch belongs to (the first level it does for a block kernel).
 struct ProfRec {
   int kind, level;
@@ -256,6 +292,132 @@ static void init_once() {
 }
 
 static size_t pad_pitch(size_t w) { return (w + 63) & ~size_t(63); }
+
+// Two-element vector access is possible: pointer aligned to a pair, even pitch.
+template <typename T>
+static bool aligned_pair_host(const void* p, long long pitch) {
+  return (reinterpret_cast<uintptr_t>(p) % (2 * sizeof(T))) == 0 && (pitch % 2) == 0;
+}
+
+// ---- pyramid launches ---------------------------------------------------------------
+// Schedule of the levels after level 0: 0 = one launch per level plus the
+// block kernel for the small ones (default), 1 = one pyramid launch.
+static int g_schedule = 0;
+static constexpr size_t kMaxPlans = 8;
+
+static unsigned long long spin_limit_ticks() {
+  static std::once_flag once;
+  static unsigned long long ticks = 200000000ULL;  // 2 s at 100 MHz
+  std::call_once(once, [] {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+      ticks = 2000ULL * (unsigned long long)khz;  // 2 s
+    else
+      (void)hipGetLastError();
+  });
+  return ticks;
+}
+
+// Run `tasks` (in producer-before-consumer order) as one pyramid launch on st.
+template <typename T>
+static void run_pyr(bool fwd, std::vector<PyrTask>& tasks, int S, bool vec, const Coeffs<T>& cf,
+                    Scratch& sc, hipStream_t st) {
+  for (auto& t : tasks) t.S = S;
+  const int ntasks = (int)tasks.size();
+  const int nitems = pyr_layout(tasks.data(), ntasks, cf.nsteps);
+  if (nitems == 0) return;
+  const int ncnt = tasks.back().cnt0 + tasks.back().nseg;
+  PyrPlan* plan = nullptr;
+  for (size_t i = 0; i < sc.plans.size(); ++i) {
+    PyrPlan* pl = sc.plans[i];
+    if (pl->fwd == fwd && pl->vec == vec && pl->S == S && pl->nsteps == cf.nsteps &&
+        pl->tasks.size() == tasks.size() &&
+        std::memcmp(pl->tasks.data(), tasks.data(), sizeof(PyrTask) * tasks.size()) == 0) {
+      plan = pl;
+      sc.plans.erase(sc.plans.begin() + (long)i);
+      sc.plans.insert(sc.plans.begin(), plan);
+      break;
+    }
+  }
+  if (!plan) {
+    std::vector<int2> deps((size_t)ncnt, make_int2(0, -1));
+    for (const PyrTask& t : tasks)
+      if (t.dep >= 0) pyr_deps(t, tasks[t.dep], fwd, cf.nsteps, deps.data() + t.cnt0);
+    plan = new PyrPlan();
+    plan->tasks = tasks;
+    plan->fwd = fwd;
+    plan->vec = vec;
+    plan->S = S;
+    plan->nsteps = cf.nsteps;
+    plan->nitems = nitems;
+    plan->ncnt = ncnt;
+    plan->deps_off = (sizeof(PyrTask) * tasks.size() + 255) & ~size_t(255);
+    const size_t bytes = plan->deps_off + sizeof(int2) * deps.size();
+    if (hipHostMalloc(&plan->pinned, bytes, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&plan->dev, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      Scratch::free_plan(plan);
+      fail(DWT_GPU_EMEMORY, "allocating a pyramid task table of %zu bytes failed", bytes);
+    }
+    std::memcpy(plan->pinned, tasks.data(), sizeof(PyrTask) * tasks.size());
+    std::memcpy((char*)plan->pinned + plan->deps_off, deps.data(), sizeof(int2) * deps.size());
+    HIPCHK(hipMemcpyAsync(plan->dev, plan->pinned, bytes, hipMemcpyHostToDevice, st));
+    sc.plans.insert(sc.plans.begin(), plan);
+    if (sc.plans.size() > kMaxPlans) {
+      HIPCHK(hipStreamSynchronize(st));  // the evicted plan may still be in use
+      Scratch::free_plan(sc.plans.back());
+      sc.plans.pop_back();
+    }
+  }
+  if (sc.sync_cnt < ncnt) {
+    if (sc.sync) {
+      HIPCHK(hipStreamSynchronize(st));
+      HIPCHK(hipFree(sc.sync));
+      sc.sync = nullptr;
+      sc.sync_cnt = 0;
+    }
+    const long long cap = std::max<long long>(ncnt, 4096);
+    if (hipMalloc(&sc.sync, sizeof(unsigned) * (size_t)(4 + cap)) != hipSuccess) {
+      (void)hipGetLastError();
+      sc.sync = nullptr;
+      fail(DWT_GPU_EMEMORY, "allocating pyramid counters failed");
+    }
+    sc.sync_cnt = cap;
+    LAUNCH(launch_fill_u32(sc.sync, 0u, 4 + cap, st));  // a kernel, not a memset: capturable
+  }
+  PyrSync sy{};
+  sy.ctrl = sc.sync;
+  sy.err = sc.sync + 2;
+  sy.cnt = sc.sync + 4;
+  sy.deps = reinterpret_cast<const int2*>(static_cast<char*>(plan->dev) + plan->deps_off);
+  sy.ncnt = ncnt;
+  sy.spin_limit = spin_limit_ticks();
+  sc.pyr_used = true;
+  LAUNCH(launch_pyr<T>(fwd, vec, S, static_cast<const PyrTask*>(plan->dev), ntasks, nitems, sy, cf,
+                       0, st));
+}
+
+// Reads and clears the pyramid error flag of sc (synchronises st); true when a
+// dependency wait of an earlier launch timed out.
+static bool pyr_failed(Scratch& sc, hipStream_t st) {
+  if (!sc.pyr_used || !sc.sync) return false;
+  if (!sc.err_host) HIPCHK(hipHostMalloc((void**)&sc.err_host, sizeof(unsigned), hipHostMallocDefault));
+  HIPCHK(hipMemcpyAsync(sc.err_host, sc.sync + 2, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (*sc.err_host == 0) return false;
+  LAUNCH(launch_fill_u32(sc.sync + 2, 0u, 1, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return true;
+}
+
+// Segment rows of a pyramid launch whose largest level is W x H: the
+// single-shot shape of the tile table, else 8.
+static int pyr_rows(bool fwd, int W, int H, int nsteps) {
+  int S = 8, U = 0;
+  tile_get(fwd ? 1 : 0, H, W, nsteps, &S, &U);
+  return (U == 0 && (S == 8 || S == 16 || S == 32)) ? S : 8;
+}
 
 // ---- generic path: per-level line passes in place on any byte strides --------
 template <typename T>
@@ -311,6 +473,124 @@ static void generic_inv_level(char* ptr, long long sx, long long sy, int Wo, int
 //  * Level 1 <= j < jt reads LL_j from ping-pong scratch P[j % 2] and writes
 //    its details straight into A (that part of A was consumed by level 0) and
 //    LL_{j+1} to P[(j + 1) % 2] (or to A when it is the last level).
+// Pyramid schedule.  Level 0 is its own launch as above (A -> LL_1 in P[1],
+// details in D, copied into A on the side stream); levels 1 .. L-1 are one
+// pyramid launch: level j reads LL_j (P[1] for j = 1, else Q_j), writes its
+// details into A and LL_{j+1} to Q_{j+1}, the last level's LL into A.  The
+// Q_j (j >= 2) are distinct buffers, so no memory is reused inside the launch.
+template <typename T>
+struct PyrBuffers {
+  T* q[32] = {};
+  long long qp[32] = {};
+};
+template <typename T>
+static PyrBuffers<T> pyr_buffers(int W, int H, int L, Scratch& sc) {
+  PyrBuffers<T> b;
+  size_t off[32] = {}, total = 0;
+  for (int j = 2; j < L; ++j) {
+    b.qp[j] = (long long)pad_pitch(ceil_div_pow2(W, j));
+    off[j] = total;
+    total += sizeof(T) * (size_t)b.qp[j] * ceil_div_pow2(H, j);
+    total = (total + 255) & ~size_t(255);
+  }
+  if (total) {
+    char* base = (char*)sc.get(5, total);
+    for (int j = 2; j < L; ++j) b.q[j] = (T*)(base + off[j]);
+  }
+  return b;
+}
+
+template <typename T>
+static void fast_fwd_pyr(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, Scratch& sc,
+                         hipStream_t st) {
+  const int W1 = ceil_div_pow2(W, 1), H1 = ceil_div_pow2(H, 1);
+  const long long Dp = (long long)pad_pitch(W);
+  T* D = (T*)sc.get(2, sizeof(T) * (size_t)Dp * H);
+  const long long P1p = (long long)pad_pitch(W1);
+  T* P1 = L >= 2 ? (T*)sc.get(1, sizeof(T) * (size_t)P1p * H1) : nullptr;
+  const PyrBuffers<T> q = pyr_buffers<T>(W, H, L, sc);
+  sc.ensure_side();
+  {
+    ProfScope ps(0, 0, st);
+    LAUNCH(launch_fwd_level<T>(A, p, W, H, L >= 2 ? P1 : D, L >= 2 ? P1p : Dp, D, Dp, cf, st));
+  }
+  HIPCHK(hipEventRecord(sc.fork, st));
+  HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
+  {
+    ProfScope ps(2, 0, sc.side);
+    LAUNCH(launch_copy_rect<T>(D + W1, Dp, A + W1, p, W - W1, H1, sc.side);
+           launch_copy_rect<T>(D + (long long)H1 * Dp, Dp, A + (long long)H1 * p, p, W, H - H1, sc.side);
+           if (L == 1) launch_copy_rect<T>(D, Dp, A, p, W1, H1, sc.side));
+  }
+  if (L >= 2) {
+    std::vector<PyrTask> tasks;
+    for (int j = 1; j < L; ++j) {
+      PyrTask t{};
+      t.src = j == 1 ? (const void*)P1 : (const void*)q.q[j];
+      t.sp = j == 1 ? P1p : q.qp[j];
+      t.out = j == L - 1 ? (void*)A : (void*)q.q[j + 1];
+      t.op = j == L - 1 ? p : q.qp[j + 1];
+      t.det = A;
+      t.dp = p;
+      t.W = ceil_div_pow2(W, j);
+      t.H = ceil_div_pow2(H, j);
+      t.dep = j == 1 ? -1 : (int)tasks.size() - 1;
+      tasks.push_back(t);
+    }
+    ProfScope ps(6, 1, st);
+    run_pyr<T>(true, tasks, pyr_rows(true, W1, H1, cf.nsteps), aligned_pair_host<T>(A, p), cf, sc, st);
+  }
+  HIPCHK(hipEventRecord(sc.join, sc.side));
+  HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
+}
+
+// Inverse of fast_fwd_pyr: A's level-0 details go to D on the side stream;
+// one pyramid launch does levels L-1 .. 1 (LL_L from A, outputs in Q_j, LL_1
+// in P[1]); then the level-0 inverse writes the image into A.
+template <typename T>
+static void fast_inv_pyr(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, Scratch& sc,
+                         hipStream_t st) {
+  const int W1 = ceil_div_pow2(W, 1), H1 = ceil_div_pow2(H, 1);
+  const long long Dp = (long long)pad_pitch(W);
+  T* D = (T*)sc.get(2, sizeof(T) * (size_t)Dp * H);
+  const long long P1p = (long long)pad_pitch(W1);
+  T* P1 = L >= 2 ? (T*)sc.get(1, sizeof(T) * (size_t)P1p * H1) : nullptr;
+  const PyrBuffers<T> q = pyr_buffers<T>(W, H, L, sc);
+  sc.ensure_side();
+  HIPCHK(hipEventRecord(sc.fork, st));
+  HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
+  {
+    ProfScope ps(2, 0, sc.side);
+    LAUNCH(launch_copy_rect<T>(A + W1, p, D + W1, Dp, W - W1, H1, sc.side);
+           launch_copy_rect<T>(A + (long long)H1 * p, p, D + (long long)H1 * Dp, Dp, W, H - H1, sc.side);
+           if (L == 1) launch_copy_rect<T>(A, p, D, Dp, W1, H1, sc.side));
+  }
+  HIPCHK(hipEventRecord(sc.join, sc.side));
+  if (L >= 2) {
+    std::vector<PyrTask> tasks;
+    for (int j = L - 1; j >= 1; --j) {
+      PyrTask t{};
+      t.src = j == L - 1 ? (const void*)A : (const void*)q.q[j + 1];
+      t.sp = j == L - 1 ? p : q.qp[j + 1];
+      t.out = j == 1 ? (void*)P1 : (void*)q.q[j];
+      t.op = j == 1 ? P1p : q.qp[j];
+      t.det = A;
+      t.dp = p;
+      t.W = ceil_div_pow2(W, j);
+      t.H = ceil_div_pow2(H, j);
+      t.dep = j == L - 1 ? -1 : (int)tasks.size() - 1;
+      tasks.push_back(t);
+    }
+    ProfScope ps(7, 1, st);
+    run_pyr<T>(false, tasks, pyr_rows(false, W1, H1, cf.nsteps), aligned_pair_host<T>(A, p), cf, sc, st);
+  }
+  HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
+  {
+    ProfScope ps(1, 0, st);
+    LAUNCH(launch_inv_level<T>(L >= 2 ? P1 : D, L >= 2 ? P1p : Dp, D, Dp, A, p, W, H, cf, st));
+  }
+}
+
 template <typename T>
 static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, Scratch& sc,
                      hipStream_t st) {
@@ -318,6 +598,10 @@ static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
   if (jt == 0) {
     ProfScope ps(4, 0, st);
     LAUNCH(launch_block_fwd<T>(A, p, A, p, W, H, L, cf, st));
+    return;
+  }
+  if (g_schedule == 1) {
+    fast_fwd_pyr<T>(A, p, W, H, L, cf, sc, st);
     return;
   }
   const int W1 = ceil_div_pow2(W, 1), H1 = ceil_div_pow2(H, 1);
@@ -381,6 +665,10 @@ static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
   if (jt == 0) {
     ProfScope ps(5, 0, st);
     LAUNCH(launch_block_inv<T>(A, p, A, p, W, H, L, cf, st));
+    return;
+  }
+  if (g_schedule == 1) {
+    fast_inv_pyr<T>(A, p, W, H, L, cf, sc, st);
     return;
   }
   const int W1 = ceil_div_pow2(W, 1), H1 = ceil_div_pow2(H, 1);
@@ -533,7 +821,10 @@ static void run2d(bool fwd, void* ptr, int sx, int sy, int Wo, int Ho, int Wi, i
       fwd2_device<T>((char*)dptr, sx, sy, Wo, Ho, Wi, Hi, jp, d1, zp, w, st);
     else
       inv2_device<T>((char*)dptr, sx, sy, Wo, Ho, Wi, Hi, *jp, d1, zp, w, st);
-    if (sync) HIPCHK(hipStreamSynchronize(st));
+    if (sync) {
+      HIPCHK(hipStreamSynchronize(st));
+      if (pyr_failed(scratch_for(st), st)) fail(DWT_GPU_EHIP, "pyramid dependency wait timed out");
+    }
     return;
   }
   // pageable host memory: stage the outer frame as a packed device image
@@ -565,8 +856,9 @@ static void run2d(bool fwd, void* ptr, int sx, int sy, int Wo, int Ho, int Wi, i
   } else {
     HIPCHK(hipMemcpyAsync(host.data(), dev, pitch * Ho, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    host_scatter<T>(host, (char*)ptr, sx, sy, Wo, Ho);
   }
+  if (pyr_failed(sc, st)) fail(DWT_GPU_EHIP, "pyramid dependency wait timed out");
+  if (!rows) host_scatter<T>(host, (char*)ptr, sx, sy, Wo, Ho);
 }
 
 // One level of one strided line (src/libdwt.c:5512-5603, 5779-5870).
@@ -846,6 +1138,27 @@ int dwt_gpu_tile_get(int forward, int width, int height, int wavelet, int* seg_r
                      int* prefetch_pairs) {
   if (!seg_rows || !prefetch_pairs || width < 1 || height < 1) return DWT_GPU_EINVAL;
   return tile_get(forward, height, width, wavelet == DWT_GPU_CDF53 ? 2 : 4, seg_rows, prefetch_pairs);
+}
+
+int dwt_gpu_schedule_set(int mode) {
+  if (mode != 0 && mode != 1) {
+    g_err = "dwt_gpu_schedule_set: mode must be 0 or 1";
+    return DWT_GPU_EINVAL;
+  }
+  g_schedule = mode;
+  return DWT_GPU_OK;
+}
+
+int dwt_gpu_device_check(void* stream) {
+  try {
+    g_err.clear();
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(hipStreamSynchronize(st));
+    if (pyr_failed(scratch_for(st), st)) fail(DWT_GPU_EHIP, "pyramid dependency wait timed out");
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
 }
 
 int dwt_gpu_tail_set(long long max_region_bytes) {

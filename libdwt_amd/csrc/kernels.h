@@ -55,4 +55,49 @@ template <typename T>
 void launch_copy_rect(const T* src, long long sp, T* dst, long long dp, int w, int h,
                       hipStream_t st);
 
+// ---- pyramid kernels (pyr_kernels.hip) ----------------------------------------
+// One level of one image in a pyramid launch.  Forward: src = level input
+// (W x H), out = LL of the next level, det = detail subbands at their Mallat
+// positions.  Inverse: src = LL of the coarser level, det = details at their
+// Mallat positions of the W x H output region, out = that region.  Pitches in
+// elements.  S = segment rows; nstrips .. cnt0 are filled by pyr_layout; dep
+// is the index of the task that produces src in the same launch (-1: none;
+// it must precede this task in the table).
+struct PyrTask {
+  const void* src;
+  long long sp;
+  void* out;
+  long long op;
+  void* det;
+  long long dp;
+  int W, H, S;
+  int nstrips, ngroups, nseg;
+  int item0, cnt0;
+  int dep, reserved;
+};
+// Device synchronisation state of a launch: ctrl[0] next item, ctrl[1]
+// finished workgroups; cnt[ncnt] finished workgroups per (task, segment), all
+// zero before a launch (the launch leaves them zero); deps[ncnt] producer
+// segment range of each (task, segment); err set by a wait that ran longer
+// than spin_limit ticks of the constant-rate clock.
+struct PyrSync {
+  unsigned* ctrl;
+  unsigned* cnt;
+  const int2* deps;
+  unsigned* err;
+  int ncnt;
+  unsigned long long spin_limit;
+};
+// Assign items and counters to tasks (in table order); returns the item count.
+int pyr_layout(PyrTask* tasks, int ntasks, int nsteps);
+// Producer segment range of every segment of task t (host).
+void pyr_deps(const PyrTask& t, const PyrTask& producer, bool fwd, int nsteps, int2* out);
+// Launch a pyramid over the device task table; all tasks use segment rows S
+// (8, 16 or 32); vec: every src/out pointer and pitch allows two-element
+// vector access.  grid_cap > 0 limits the number of workgroups.
+template <typename T>
+void launch_pyr(bool fwd, bool vec, int S, const PyrTask* d_tasks, int ntasks, int nitems,
+                const PyrSync& sy, const Coeffs<T>& cf, int grid_cap, hipStream_t st);
+void launch_fill_u32(unsigned* p, unsigned v, long long n, hipStream_t st);
+
 }  // namespace dwt

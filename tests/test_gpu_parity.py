@@ -48,7 +48,7 @@ def device():
     return torch.device("cuda:0")
 
 
-def dropin(mode, timeout=240):
+def dropin(mode, timeout=140):
     """Run the drop-in entry points over the fixture cases in a child process:
     they abort on error like the reference, and a child keeps that abort (and
     the library's message, and the case it names) contained and visible."""
@@ -168,4 +168,71 @@ def test_async_api_on_torch_stream(gold):
         libdwt_amd.transform_2d_async("cdf97", True, t.data_ptr(), 4 * cfg["W"], 4, cfg["W"],
                                       cfg["H"], j=cfg["J"], stream=s.cuda_stream)
     s.synchronize()
+    np.testing.assert_array_equal(t.cpu().numpy(), want)
+
+
+PYR_SHAPES = [(2, 2), (3, 5), (17, 33), (130, 70), (300, 257), (1000, 40), (9, 250), (513, 511),
+              (2048, 3)]
+
+
+@pytest.mark.parametrize("prec", ["s", "d"])
+def test_pyramid_schedule_shapes(prec):
+    """Level 0 plus one pyramid (dataflow) launch for every later level, with
+    the single-workgroup block kernel switched off so that small images take
+    this path too: bit-exact to the oracle and equal to the per-level
+    schedule, forward and inverse, both wavelets."""
+    from oracle.oracle import oracle
+    o = oracle()
+    lib = libdwt_amd.lib()
+    dt = np.float32 if prec == "s" else np.float64
+    rng = np.random.default_rng(11)
+    st = torch.cuda.current_stream().cuda_stream
+    try:
+        assert lib.dwt_gpu_tail_set(0) == 0
+        for (W, H) in PYR_SHAPES:
+            for w in cases.WAVELETS:
+                img = rng.standard_normal((H, W)).astype(dt)
+                ref, J = o.fwd2(img, w, -1)
+                want = o.inv2(ref, w, J)
+                for sched in (1, 0):
+                    say(f"pyramid {W}x{H} {w} {prec} schedule {sched}")
+                    libdwt_amd.schedule_set(sched)
+                    t = torch.from_numpy(img).cuda()
+                    j = libdwt_amd.fwd2(t, w, -1)
+                    libdwt_amd.device_check(st)
+                    assert j == J
+                    np.testing.assert_array_equal(t.cpu().numpy(), ref, err_msg=f"forward {sched}")
+                    libdwt_amd.inv2(t, w, j)
+                    libdwt_amd.device_check(st)
+                    np.testing.assert_array_equal(t.cpu().numpy(), want, err_msg=f"inverse {sched}")
+    finally:
+        lib.dwt_gpu_tail_set(16 * 1024)
+        libdwt_amd.schedule_set(0)
+
+
+def test_pyramid_repeated_launches_reuse_counters():
+    """Forty back-to-back forward+inverse pairs on one stream reuse the
+    pyramid's counters (each launch leaves them zero for the next); the
+    result equals the oracle doing the same forty round trips."""
+    from oracle.oracle import oracle
+    say("pyramid repeated launches")
+    o = oracle()
+    img = workloads.sinusoid_image(640, 480, 9)
+    want = img
+    for _ in range(40):
+        c, j = o.fwd2(want, "cdf53", 6)
+        want = o.inv2(c, "cdf53", j)
+    s = torch.cuda.Stream()
+    t = torch.from_numpy(img).cuda()
+    torch.cuda.synchronize()
+    libdwt_amd.schedule_set(1)
+    try:
+        for _ in range(40):
+            j = libdwt_amd.transform_2d_async("cdf53", True, t.data_ptr(), 4 * 640, 4, 640, 480,
+                                              j=6, stream=s.cuda_stream)
+            libdwt_amd.transform_2d_async("cdf53", False, t.data_ptr(), 4 * 640, 4, 640, 480,
+                                          j=j, stream=s.cuda_stream)
+        libdwt_amd.device_check(s.cuda_stream)
+    finally:
+        libdwt_amd.schedule_set(0)
     np.testing.assert_array_equal(t.cpu().numpy(), want)
