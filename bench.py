@@ -58,6 +58,9 @@ def parse():
                     help="levels after level 0: 1 one pyramid launch, 0 one launch per level")
     ap.add_argument("--cold-steps", type=int, default=10,
                     help="steps timed one at a time after a 512 MiB cache scrub")
+    ap.add_argument("--workload", default="image", choices=("image", "batch"),
+                    help="image: one 4096x4096 image (the headline metric); batch: 256 "
+                         "images of uneven sizes, batched launches vs one launch per image")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -106,6 +109,166 @@ def cpu_baseline(img: np.ndarray, wavelet: str, levels: int, budget_s: float) ->
             "kind": kind, "sample": sample, "seconds_per_step": best}
 
 
+def _timed(torch, stream, fn, steps):
+    """ms per call of fn() over `steps` calls on `stream` (device-synchronised)."""
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def bench_batch(args, torch, libdwt_amd, workloads):
+    """256 float32 images, each side log-uniform in [64, 2048] (seed 4),
+    J = min(8, floor(log2(min side))), forward then inverse: batched launches
+    (FP32 / FP16 / BF16 coefficient storage, eagerly and as a replayed HIP
+    graph) against one drop-in transform per image.  Errors are measured
+    against the library's own FP64 transform (bit-identical to the reference's
+    _d path) on the first 32 images."""
+    import ctypes
+    import math
+    wv = args.wavelet
+    sizes = workloads.uneven_sizes(256, 64, 2048, seed=4)
+    levels = [min(8, int(math.floor(math.log2(min(w, h))))) for w, h in sizes]
+    area = sum(w * h for w, h in sizes)
+    dev = torch.device("cuda:0")
+    imgs = [torch.from_numpy(workloads.sinusoid_image(w, h, seed=1000 + i)).to(dev)
+            for i, (w, h) in enumerate(sizes)]
+    stream = torch.cuda.Stream(device=dev)
+    sp = stream.cuda_stream
+    L = libdwt_amd.lib()
+    steps, warm = max(1, args.steps), max(1, args.warmup)
+
+    # one drop-in style transform per image (in place on working copies)
+    work = [t.clone() for t in imgs]
+
+    def plain_step():
+        for t, j in zip(work, levels):
+            H, W = t.shape
+            jj = libdwt_amd.transform_2d_async(wv, True, t.data_ptr(), 4 * W, 4, W, H, j=j, stream=sp)
+            libdwt_amd.transform_2d_async(wv, False, t.data_ptr(), 4 * W, 4, W, H, j=jj, stream=sp)
+
+    for _ in range(min(warm, 2)):
+        plain_step()
+    plain_ms = _timed(torch, stream, plain_step, min(steps, 5))
+    del work
+
+    # FP64 reference coefficients of the first images (the GPU's own _d path)
+    nerr = 32
+    ref64 = []
+    with torch.cuda.stream(stream):  # conversion and transform ordered on one stream
+        for t, j in zip(imgs[:nerr], levels[:nerr]):
+            d = t.double()
+            H, W = t.shape
+            libdwt_amd.transform_2d_async(wv, True, d.data_ptr(), 8 * W, 8, W, H, j=j, f64=True,
+                                          stream=sp)
+            ref64.append(d)
+    torch.cuda.synchronize()
+
+    def scale_map(s, H, W, J):
+        m = torch.empty((H, W), dtype=torch.float64, device=dev)
+        for lev in range(J):
+            w, h = -(-W // 2 ** lev), -(-H // 2 ** lev)
+            wd, hd = -(-w // 2), -(-h // 2)
+            m[:hd, wd:w] = s[3 * lev]
+            m[hd:h, :wd] = s[3 * lev + 1]
+            m[hd:h, wd:w] = s[3 * lev + 2]
+        m[:-(-H // 2 ** J), :-(-W // 2 ** J)] = s[3 * J]
+        return m
+
+    modes = {}
+    for storage in ("f32", "f16", "bf16"):
+        dt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[storage]
+        coefs = [torch.empty(t.shape, dtype=dt, device=dev) for t in imgs]
+        outs = [torch.empty_like(t) for t in imgs]
+        scales = ([torch.zeros(3 * j + 1, device=dev) for j in levels] if storage != "f32" else None)
+        fa = libdwt_amd.batch_descriptors(imgs, coefs, levels, scales)
+        ia = libdwt_amd.batch_descriptors(coefs, outs, levels, scales)
+        pf, pi = ctypes.c_void_p(), ctypes.c_void_p()
+        sk = libdwt_amd.STORAGE[storage]
+        wid = libdwt_amd.WAVELET_IDS[wv]
+        assert L.dwt_gpu_batch_plan_create(wid, 0, sk, fa, len(imgs), sp, ctypes.byref(pf)) == 0, \
+            L.dwt_gpu_last_error()
+        assert L.dwt_gpu_batch_plan_create(wid, 1, sk, ia, len(imgs), sp, ctypes.byref(pi)) == 0, \
+            L.dwt_gpu_last_error()
+
+        def step():
+            L.dwt_gpu_batch_plan_execute(pf, sp)
+            L.dwt_gpu_batch_plan_execute(pi, sp)
+
+        for _ in range(warm):
+            step()
+        ms = _timed(torch, stream, step, steps)
+        # the same two plan executions captured in a HIP graph and replayed
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=stream):
+            step()
+        with torch.cuda.stream(stream):  # replay() launches on the current stream
+            g.replay()
+            ms_graph = _timed(torch, stream, g.replay, steps)
+        # per-level kernel times of one eager step
+        libdwt_amd.profile_reset()
+        libdwt_amd.profile_enable(True)
+        step()
+        torch.cuda.synchronize()
+        libdwt_amd.profile_enable(False)
+        kern = {}
+        for kind, name in ((8, "fwd"), (9, "inv")):
+            for lev in range(8):
+                n, t = libdwt_amd.profile_read(kind, lev)
+                if n:
+                    kern[f"{name}{lev}"] = round(1000 * t / n, 2)
+        libdwt_amd.profile_reset()
+        # errors on the first images
+        cerr, rerr, psnr = 0.0, 0.0, float("inf")
+        for k in range(nerr):
+            H, W = imgs[k].shape
+            c = coefs[k].double()
+            if scales is not None:
+                c = c / scale_map(scales[k].double(), H, W, levels[k])
+            cerr = max(cerr, float((c - ref64[k]).abs().max()))
+            x = imgs[k].double()
+            e = outs[k].double() - x
+            rerr = max(rerr, float(e.abs().max()))
+            peak = float(x.max() - x.min())
+            mse = float((e * e).mean())
+            psnr = min(psnr, 10 * math.log10(peak * peak / mse) if mse > 0 else float("inf"))
+        L.dwt_gpu_batch_plan_destroy(pf)
+        L.dwt_gpu_batch_plan_destroy(pi)
+        del g
+        modes[storage] = {
+            "ms_per_step": round(ms, 4), "Mpix_s": round(area / (ms * 1e-3) / 1e6, 1),
+            "ms_per_step_graph": round(ms_graph, 4),
+            "Mpix_s_graph": round(area / (ms_graph * 1e-3) / 1e6, 1),
+            "coef_max_abs_err_vs_fp64": cerr, "recon_max_abs_err": rerr,
+            "recon_psnr_min_db": round(psnr, 2), "level_kernels_us": kern,
+        }
+    alg = sum(2 * workloads.algorithmic_bytes(w, h, j) for (w, h), j in zip(sizes, levels))
+    f32 = modes["f32"]
+    best_ms = min(f32["ms_per_step"], f32["ms_per_step_graph"])
+    out = {
+        "metric": f"Mpix/s, 256 uneven images (sides 64..2048) {wv} batched fwd+inv",
+        "value": round(area / (best_ms * 1e-3) / 1e6, 1),
+        "unit": "Mpix/s",
+        "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": round(best_ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (sinusoid+noise images, seeds 1000..1255; sizes seed 4)",
+        "config": {"workload": "batch", "images": len(sizes), "pixels": area,
+                   "levels": "min(8, floor(log2(min side)))", "wavelet": wv},
+        "plain_per_image": {"ms_per_step": round(plain_ms, 3),
+                            "Mpix_s": round(area / (plain_ms * 1e-3) / 1e6, 1)},
+        "speedup_vs_plain": round(plain_ms / best_ms, 2),
+        "hbm_frac_f32": round(alg / (best_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "modes": modes,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -131,6 +294,9 @@ def main():
     lib = libdwt_amd.lib()
     assert lib.dwt_gpu_init() == 0, lib.dwt_gpu_last_error()
     libdwt_amd.schedule_set(args.schedule)
+    if args.workload == "batch":
+        bench_batch(args, torch, libdwt_amd, workloads)
+        return
 
     W, H, J, wv = args.width, args.height, args.levels, args.wavelet
     img = workloads.sinusoid_image(W, H, seed=3)

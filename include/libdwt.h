@@ -135,7 +135,8 @@ int dwt_gpu_2d_async(int wavelet, int direction, int precision, void *ptr, int s
 /* Per-launch timing with HIP events recorded on the launch stream around each
  * kernel of the packed path.  kind: 0 forward level kernel, 1 inverse level
  * kernel, 2 level-0 detail copy, 4 forward block kernel, 5 inverse block
- * kernel, 6 forward pyramid, 7 inverse pyramid (levels after level 0);
+ * kernel, 6 forward pyramid, 7 inverse pyramid (levels after level 0),
+ * 8 / 9 forward / inverse batched level launch;
  * level: the level index, the first level it does for a block kernel or a
  * pyramid (-1 = all).
  * profile_read synchronises on the recorded events. */
@@ -170,7 +171,63 @@ int dwt_gpu_schedule_set(int mode);
  * themselves before returning. */
 int dwt_gpu_device_check(void *stream);
 
-/* Bytes of device scratch the calling thread currently holds for `stream`. */
+/* Timeline of pyramid launches: with max_items > 0 every later pyramid launch
+ * records, per work item, {start, dependencies met, done} in ticks of the
+ * GPU's constant-rate clock (dwt_gpu_clock_rate_khz) and the index of the
+ * workgroup that ran it.  dwt_gpu_pyr_trace_read copies the records of the
+ * last launch on `stream` (synchronising it) and returns their count. */
+int dwt_gpu_pyr_trace_enable(int max_items);
+/* Rate of the GPU's constant-rate clock in kHz. */
+int dwt_gpu_clock_rate_khz(void);
+/* At most max_workgroups workgroups per pyramid launch (0: as many as fit). */
+int dwt_gpu_pyr_grid_cap_set(int max_workgroups);
+long long dwt_gpu_pyr_trace_read(void *stream, unsigned long long *out, long long max_items);
+
+/* ---- batches of images of uneven sizes ------------------------------------
+ * One call transforms many packed single-channel images, out of place, with
+ * one launch per level for the whole batch.  Each image: src -> dst;
+ * forward: src is the FP32 image, dst receives its coefficients in Mallat
+ * layout (the layout of dwt_cdf97_2f_s with size_o == size_i); inverse: src
+ * holds the coefficients, dst receives the FP32 image.  The coefficients are
+ * stored as FP32 (bit-identical to the single-image transform), FP16 or BF16;
+ * the lifting arithmetic is FP32.  For 16-bit storage every subband is stored
+ * multiplied by a power-of-two scale that the forward chooses from the
+ * image's max |x| (so that |coefficient * scale| <= 2^14) and writes to
+ * `scales` (3 per level: HL, LH, HH, then the final LL: 3 * levels + 1
+ * floats in device memory); the inverse reads them from there.  Pitches are
+ * in bytes, multiples of the element size.
+ * Images need width and height >= 2; levels as j_max of dwt_cdf97_2f_s with
+ * decompose_one = 0 (forward: < 0 or too large means the maximum, written
+ * back; inverse: the levels to undo). */
+typedef struct dwt_gpu_image {
+	const void *src;
+	void *dst;
+	long long src_pitch;
+	long long dst_pitch;
+	int width;
+	int height;
+	int levels;
+	int reserved;
+	float *scales;
+} dwt_gpu_image;
+enum dwt_gpu_storage { DWT_GPU_STORE_F32 = 0, DWT_GPU_STORE_F16 = 1, DWT_GPU_STORE_BF16 = 2 };
+
+/* One-shot batch on `stream` (asynchronous).  The plans of recent batches
+ * (same descriptors) are kept and reused. */
+int dwt_gpu_batch_2d(int wavelet, int direction, int storage, dwt_gpu_image *images, int n,
+                     void *stream);
+/* Plans: create once (tables uploaded, scratch allocated; synchronises
+ * `stream`), execute any number of times on one stream at a time (only kernel
+ * launches, so it can be captured in a HIP graph), destroy. */
+int dwt_gpu_batch_plan_create(int wavelet, int direction, int storage, dwt_gpu_image *images,
+                              int n, void *stream, void **plan);
+int dwt_gpu_batch_plan_execute(void *plan, void *stream);
+int dwt_gpu_batch_plan_destroy(void *plan);
+/* Segment rows (8, 16 or 32; 0: default 8) of the batched launches of one
+ * level, for plans created afterwards. */
+int dwt_gpu_batch_rows_set(int level, int seg_rows);
+
+/* Bytes of device scratch the calling thread currently holds for `stream`. *//* Bytes of device scratch the calling thread currently holds for `stream`. */
 long long dwt_gpu_scratch_bytes(void *stream);
 /* Release the calling thread's scratch for `stream` (synchronises it). */
 int dwt_gpu_release_scratch(void *stream);

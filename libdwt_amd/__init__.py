@@ -24,6 +24,16 @@ WAVELET_IDS = {"cdf97": CDF97, "cdf53": CDF53}
 _lib = None
 
 
+class BatchImage(C.Structure):
+    """struct dwt_gpu_image."""
+    _fields_ = [("src", C.c_void_p), ("dst", C.c_void_p), ("src_pitch", C.c_longlong),
+                ("dst_pitch", C.c_longlong), ("width", C.c_int), ("height", C.c_int),
+                ("levels", C.c_int), ("reserved", C.c_int), ("scales", C.c_void_p)]
+
+
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
+
+
 class LibraryMissing(RuntimeError):
     pass
 
@@ -93,6 +103,24 @@ def lib() -> C.CDLL:
         L.dwt_gpu_schedule_set.restype = i
         L.dwt_gpu_device_check.argtypes = [vp]
         L.dwt_gpu_device_check.restype = i
+        L.dwt_gpu_pyr_trace_enable.argtypes = [i]
+        L.dwt_gpu_pyr_trace_enable.restype = i
+        L.dwt_gpu_pyr_trace_read.argtypes = [vp, vp, C.c_longlong]
+        L.dwt_gpu_pyr_trace_read.restype = C.c_longlong
+        L.dwt_gpu_clock_rate_khz.restype = i
+        L.dwt_gpu_pyr_grid_cap_set.argtypes = [i]
+        L.dwt_gpu_pyr_grid_cap_set.restype = i
+        L.dwt_gpu_batch_2d.argtypes = [i, i, i, C.POINTER(BatchImage), i, vp]
+        L.dwt_gpu_batch_2d.restype = i
+        L.dwt_gpu_batch_plan_create.argtypes = [i, i, i, C.POINTER(BatchImage), i, vp,
+                                                C.POINTER(vp)]
+        L.dwt_gpu_batch_plan_create.restype = i
+        L.dwt_gpu_batch_plan_execute.argtypes = [vp, vp]
+        L.dwt_gpu_batch_plan_execute.restype = i
+        L.dwt_gpu_batch_plan_destroy.argtypes = [vp]
+        L.dwt_gpu_batch_plan_destroy.restype = i
+        L.dwt_gpu_batch_rows_set.argtypes = [i, i]
+        L.dwt_gpu_batch_rows_set.restype = i
         _lib = L
     return _lib
 
@@ -151,6 +179,87 @@ def schedule_set(mode: int) -> None:
 def device_check(stream: int = 0) -> None:
     """Raise if a pyramid launch on `stream` (this thread) hit its wait limit."""
     _check(lib().dwt_gpu_device_check(stream))
+
+
+def pyr_trace(stream: int = 0, max_items: int = 1 << 20):
+    """Per-item timeline of the last pyramid launch on `stream`: an int64
+    array of rows (start, dependencies met, done, workgroup), clock ticks in
+    microseconds from the first start."""
+    import numpy as np
+    buf = np.zeros((max_items, 4), dtype=np.uint64)
+    n = lib().dwt_gpu_pyr_trace_read(stream, buf.ctypes.data, max_items)
+    if n < 0:
+        _check(int(n))
+    rows = buf[:n].astype(np.int64)
+    khz = lib().dwt_gpu_clock_rate_khz()
+    t0 = rows[:, 0].min() if n else 0
+    out = rows.astype(np.float64)
+    out[:, :3] = (rows[:, :3] - t0) * (1000.0 / khz)
+    return out
+
+
+def _storage_dtype(storage: str):
+    import torch
+    return {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[storage]
+
+
+def batch_descriptors(srcs, dsts, levels, scales=None):
+    """dwt_gpu_image array for lists of 2D CUDA tensors (packed rows)."""
+    n = len(srcs)
+    arr = (BatchImage * n)()
+    for k, (a, b) in enumerate(zip(srcs, dsts)):
+        H, W = a.shape
+        arr[k].src = a.data_ptr()
+        arr[k].dst = b.data_ptr()
+        arr[k].src_pitch = a.stride(0) * a.element_size()
+        arr[k].dst_pitch = b.stride(0) * b.element_size()
+        arr[k].width = W
+        arr[k].height = H
+        arr[k].levels = levels[k] if isinstance(levels, (list, tuple)) else levels
+        arr[k].scales = scales[k].data_ptr() if scales is not None else None
+    return arr
+
+
+def batch_forward(images, wavelet: str = "cdf97", storage: str = "f32", levels=-1,
+                  stream: int | None = None):
+    """Forward transform of a list of 2D float32 CUDA tensors in one batch on
+    torch's current stream (or `stream`).  Returns (coefficients, scales,
+    levels): coefficient tensors in the storage dtype, per-image scale tensors
+    (3 * levels + 1 floats; None for f32) and the levels done per image."""
+    import torch
+    dt = _storage_dtype(storage)
+    coefs = [torch.empty(t.shape, dtype=dt, device=t.device) for t in images]
+    scales = None
+    if storage != "f32":
+        lv = levels if isinstance(levels, (list, tuple)) else [levels] * len(images)
+        scales = [torch.zeros(3 * (_max_levels(t) if l < 0 else min(l, _max_levels(t))) + 1,
+                              dtype=torch.float32, device=t.device) for t, l in zip(images, lv)]
+    arr = batch_descriptors(images, coefs, levels, scales)
+    st = torch.cuda.current_stream().cuda_stream if stream is None else stream
+    _check(lib().dwt_gpu_batch_2d(WAVELET_IDS[wavelet], FORWARD, STORAGE[storage], arr,
+                                  len(images), st))
+    return coefs, scales, [arr[k].levels for k in range(len(images))]
+
+
+def batch_inverse(coefs, levels, wavelet: str = "cdf97", storage: str = "f32", scales=None,
+                  stream: int | None = None):
+    """Inverse of batch_forward: a list of float32 image tensors."""
+    import torch
+    outs = [torch.empty(c.shape, dtype=torch.float32, device=c.device) for c in coefs]
+    arr = batch_descriptors(coefs, outs, list(levels), scales)
+    st = torch.cuda.current_stream().cuda_stream if stream is None else stream
+    _check(lib().dwt_gpu_batch_2d(WAVELET_IDS[wavelet], INVERSE, STORAGE[storage], arr, len(coefs),
+                                  st))
+    return outs
+
+
+def _max_levels(t) -> int:
+    H, W = t.shape
+    m = min(H, W)
+    r = 0
+    while (1 << r) < m:
+        r += 1
+    return r
 
 
 def profile_enable(on: bool) -> None:

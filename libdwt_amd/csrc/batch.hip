@@ -1,0 +1,289 @@
+// Batched transforms of many images of uneven sizes.
+//
+// Schedule: level-synchronous.  Level j of every image of the batch is one
+// launch: the work items (four strips x one segment of S rows, as in the
+// per-image level kernels) of all images at that level are numbered
+// image-major and a workgroup finds its image through a per-block table, so a
+// launch starts every workgroup at once with no atomics (a single ticket
+// counter shared by tens of thousands of items serialises their start; the
+// dataflow pyramid measured that).  A batch is then L launches per direction
+// for L = the deepest image's level count, whatever the number of images.
+//
+// Layout: out of place.  The forward reads each image (FP32) and writes its
+// coefficients in Mallat layout to dst; intermediate LL bands of levels
+// 1 .. L-1 live in per-image ping-pong scratch.  The inverse reads
+// coefficients from src and writes the image to dst.
+//
+// Storage: the coefficients (forward output, inverse input) are FP32, FP16 or
+// BF16; lifting arithmetic is FP32 throughout.  A 16-bit coefficient is
+// stored as round-to-nearest-even(c * s) for a per-subband power of two s
+// (exact to apply and to undo), chosen before the forward transform from the
+// image's max |x| and the filters' l1 gains so that |c * s| <= 2^14: no FP16
+// overflow, and the subband's largest coefficient uses the top of the FP16
+// exponent range.  Scales: 3 per level (HL, LH, HH) then one for the final
+// LL, in a float array per image that the inverse reads back.
+#include "tile_body.h"
+#include "kernels.h"
+#include "batch.h"
+
+#include <algorithm>
+#include <cmath>
+#include <mutex>
+
+namespace dwt {
+
+// ---- 16-bit storage ------------------------------------------------------------
+template <int SK>
+__device__ __forceinline__ unsigned short to_storage(float v) {
+  if constexpr (SK == kStoreF16) {
+    const _Float16 h = (_Float16)v;  // round to nearest even
+    return __builtin_bit_cast(unsigned short, h);
+  } else {
+    unsigned int u = __float_as_uint(v);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);  // NaN stays NaN
+    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+    return (unsigned short)(u >> 16);
+  }
+}
+template <int SK>
+__device__ __forceinline__ float from_storage(unsigned short b) {
+  if constexpr (SK == kStoreF16) return (float)__builtin_bit_cast(_Float16, b);
+  else return __uint_as_float((unsigned int)b << 16);
+}
+
+// Detail subbands in 16-bit storage with a scale per band.
+template <int SK>
+struct Det16 {
+  unsigned short* __restrict__ p;
+  long long pitch;
+  float s[3], inv[3];
+  __device__ __forceinline__ void store(int band, int r, int c, float v) const {
+    Gbl<unsigned short>* q = gstore_ptr(p);
+    q[(long long)r * pitch + c] = to_storage<SK>(v * s[band]);
+  }
+  __device__ __forceinline__ float load(int band, int r, int c) const {
+    return from_storage<SK>(gload_ptr(static_cast<const unsigned short*>(p))[(long long)r * pitch + c]) *
+           inv[band];
+  }
+};
+template <int SK>
+struct Store16 {  // LL coefficient in 16-bit storage
+  unsigned short* __restrict__ p;
+  long long pitch;
+  float s;
+  __device__ __forceinline__ void operator()(int m, int k, float v) const {
+    gstore_ptr(p)[(long long)m * pitch + k] = to_storage<SK>(v * s);
+  }
+};
+template <int SK>
+struct Load16 {
+  const unsigned short* __restrict__ p;
+  long long pitch;
+  float inv;
+  __device__ __forceinline__ float operator()(int m, int k) const {
+    return from_storage<SK>(gload_ptr(p)[(long long)m * pitch + k]) * inv;
+  }
+};
+
+// Image rows / output pairs with the two-element vector access chosen per
+// item at run time (images of one batch may or may not be pair-aligned).
+struct RowsRt {
+  GlobalRows<float, true> v;
+  bool vec;
+  __device__ __forceinline__ void operator()(int Y, int Xe, int xe, int xo, bool interior, float& e,
+                                             float& o) const {
+    v(Y, Xe, xe, xo, interior && vec, e, o);
+  }
+};
+struct OutRt {
+  GlobalOut<float, true> v;
+  GlobalOut<float, false> s;
+  bool vec;
+  __device__ __forceinline__ void operator()(int r, int Xe, float e, float o, bool st_e, bool st_o) const {
+    if (vec) v(r, Xe, e, o, st_e, st_o);
+    else s(r, Xe, e, o, st_e, st_o);
+  }
+};
+
+// ---- level kernels ------------------------------------------------------------------
+template <int NS, int SK, int SS>
+__global__ void __launch_bounds__(256)
+batch_fwd_kernel(const BatchItem* __restrict__ items, const int* __restrict__ block_item,
+                 Coeffs<float> cf) {
+  const int b = blockIdx.x;
+  const BatchItem& e = items[block_item[b]];
+  const int local = b - e.block0;
+  const int s = local / e.ngroups, g = local - s * e.ngroups;
+  const int strip = g * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (strip >= e.nstrips) return;
+  const RowsRt rows{{static_cast<const float*>(e.src), e.sp, e.H},
+                    __builtin_amdgcn_readfirstlane(e.flags & kItemVec) != 0};
+  if constexpr (SK == kStoreF32) {
+    fwd_tile<float, NS, -SS>(strip, s, SS, e.W, e.H, rows,
+                             GlobalStore<float>{static_cast<float*>(e.out), e.op},
+                             GlobalDet<float>{static_cast<float*>(e.det), e.dp}, cf);
+  } else {
+    const Gbl<float>* sc = gload_ptr(static_cast<const float*>(e.scale)) + 3 * e.level;
+    const Det16<SK> det{static_cast<unsigned short*>(e.det), e.dp, {sc[0], sc[1], sc[2]},
+                        {1.0f / sc[0], 1.0f / sc[1], 1.0f / sc[2]}};
+    if (e.flags & kItemStorageLL)  // last level: LL into the coefficients
+      fwd_tile<float, NS, -SS>(strip, s, SS, e.W, e.H, rows,
+                               Store16<SK>{static_cast<unsigned short*>(e.out), e.op,
+                                           gload_ptr(static_cast<const float*>(e.scale))[3 * e.levels]},
+                               det, cf);
+    else
+      fwd_tile<float, NS, -SS>(strip, s, SS, e.W, e.H, rows,
+                               GlobalStore<float>{static_cast<float*>(e.out), e.op}, det, cf);
+  }
+}
+
+template <int NS, int SK, int SS>
+__global__ void __launch_bounds__(256)
+batch_inv_kernel(const BatchItem* __restrict__ items, const int* __restrict__ block_item,
+                 Coeffs<float> cf) {
+  const int b = blockIdx.x;
+  const BatchItem& e = items[block_item[b]];
+  const int local = b - e.block0;
+  const int s = local / e.ngroups, g = local - s * e.ngroups;
+  const int strip = g * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (strip >= e.nstrips) return;
+  const OutRt out{{static_cast<float*>(e.out), e.op},
+                  {static_cast<float*>(e.out), e.op},
+                  __builtin_amdgcn_readfirstlane(e.flags & kItemVec) != 0};
+  if constexpr (SK == kStoreF32) {
+    inv_tile<float, NS, -SS>(strip, s, SS, e.W, e.H,
+                             GlobalLoad<float>{static_cast<const float*>(e.src), e.sp},
+                             GlobalDet<float>{static_cast<float*>(e.det), e.dp}, out, cf);
+  } else {
+    const Gbl<float>* sc = gload_ptr(static_cast<const float*>(e.scale)) + 3 * e.level;
+    const Det16<SK> det{static_cast<unsigned short*>(e.det), e.dp, {sc[0], sc[1], sc[2]},
+                        {1.0f / sc[0], 1.0f / sc[1], 1.0f / sc[2]}};
+    if (e.flags & kItemStorageLL)  // coarsest level: LL from the coefficients
+      inv_tile<float, NS, -SS>(strip, s, SS, e.W, e.H,
+                               Load16<SK>{static_cast<const unsigned short*>(e.src), e.sp,
+                                          1.0f / gload_ptr(static_cast<const float*>(e.scale))[3 * e.levels]},
+                               det, out, cf);
+    else
+      inv_tile<float, NS, -SS>(strip, s, SS, e.W, e.H,
+                               GlobalLoad<float>{static_cast<const float*>(e.src), e.sp}, det, out,
+                               cf);
+  }
+}
+
+// Max |x| of every image (atomic max on the bit pattern, which orders
+// non-negative floats); amax must be zero before.
+__global__ void __launch_bounds__(256)
+batch_maxabs_kernel(const BatchImage* __restrict__ imgs, const int* __restrict__ block_img,
+                    unsigned* __restrict__ amax) {
+  __shared__ unsigned red[4];
+  const int i = block_img[blockIdx.x];
+  const BatchImage& im = imgs[i];
+  const long long n = (long long)im.W * im.H;
+  const long long first = (long long)(blockIdx.x - im.block0) * 256 * kMaxAbsPerThread;
+  float m = 0.0f;
+  const Gbl<float>* src = gload_ptr(static_cast<const float*>(im.src));
+#pragma unroll
+  for (int u = 0; u < kMaxAbsPerThread; ++u) {
+    const long long t = first + (long long)u * 256 + threadIdx.x;
+    if (t < n) {
+      const long long y = t / im.W, x = t - y * im.W;
+      m = fmaxf(m, fabsf(src[y * im.sp + x]));
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = __float_as_uint(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned v = max(max(red[0], red[1]), max(red[2], red[3]));
+    atomicMax(amax + i, v);
+  }
+}
+
+// Per-subband scales from max |x|: s = 2^floor(log2(2^14 / (gain * max|x|))).
+__global__ void batch_scales_kernel(const BatchImage* __restrict__ imgs, int n,
+                                    const unsigned* __restrict__ amax, const float* __restrict__ gain) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const BatchImage& im = imgs[i];
+  const float mx = __uint_as_float(amax[i]);
+  for (int k = 0; k <= 3 * im.levels; ++k) {
+    // gain table: 3 per level then 32 LL entries (LL after j levels at 96 + j)
+    const float gk = k < 3 * im.levels ? gain[k] : gain[96 + im.levels];
+    const float bound = gk * mx;
+    float s = 1.0f;
+    if (bound > 0.0f && isfinite(bound)) {
+      int e = 14 - (int)ceilf(log2f(bound));
+      e = max(-60, min(60, e));
+      s = ldexpf(1.0f, e);
+    }
+    gstore_ptr(im.scale)[k] = s;
+  }
+}
+
+// ---- host launchers --------------------------------------------------------------------
+template <int NS, int SK>
+static void batch_go(bool fwd, int S, int nblocks, const BatchItem* items, const int* block_item,
+                     const Coeffs<float>& cf, hipStream_t st) {
+  const dim3 grid(nblocks), block(256);
+  if (fwd) {
+    if (S == 16) hipLaunchKernelGGL((batch_fwd_kernel<NS, SK, 16>), grid, block, 0, st, items, block_item, cf);
+    else if (S == 32) hipLaunchKernelGGL((batch_fwd_kernel<NS, SK, 32>), grid, block, 0, st, items, block_item, cf);
+    else hipLaunchKernelGGL((batch_fwd_kernel<NS, SK, 8>), grid, block, 0, st, items, block_item, cf);
+  } else {
+    if (S == 16) hipLaunchKernelGGL((batch_inv_kernel<NS, SK, 16>), grid, block, 0, st, items, block_item, cf);
+    else if (S == 32) hipLaunchKernelGGL((batch_inv_kernel<NS, SK, 32>), grid, block, 0, st, items, block_item, cf);
+    else hipLaunchKernelGGL((batch_inv_kernel<NS, SK, 8>), grid, block, 0, st, items, block_item, cf);
+  }
+}
+
+void launch_batch_level(bool fwd, int storage, int S, int nblocks, const BatchItem* items,
+                        const int* block_item, const Coeffs<float>& cf, hipStream_t st) {
+  if (nblocks <= 0) return;
+  if (cf.nsteps == 4) {
+    if (storage == kStoreF16) batch_go<4, kStoreF16>(fwd, S, nblocks, items, block_item, cf, st);
+    else if (storage == kStoreBF16) batch_go<4, kStoreBF16>(fwd, S, nblocks, items, block_item, cf, st);
+    else batch_go<4, kStoreF32>(fwd, S, nblocks, items, block_item, cf, st);
+  } else {
+    if (storage == kStoreF16) batch_go<2, kStoreF16>(fwd, S, nblocks, items, block_item, cf, st);
+    else if (storage == kStoreBF16) batch_go<2, kStoreBF16>(fwd, S, nblocks, items, block_item, cf, st);
+    else batch_go<2, kStoreF32>(fwd, S, nblocks, items, block_item, cf, st);
+  }
+}
+
+void launch_batch_maxabs(const BatchImage* imgs, const int* block_img, int nblocks, unsigned* amax,
+                         hipStream_t st) {
+  if (nblocks <= 0) return;
+  hipLaunchKernelGGL(batch_maxabs_kernel, dim3(nblocks), dim3(256), 0, st, imgs, block_img, amax);
+}
+
+void launch_batch_scales(const BatchImage* imgs, int n, const unsigned* amax, const float* gain,
+                         hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(batch_scales_kernel, dim3((n + 63) / 64), dim3(64), 0, st, imgs, n, amax, gain);
+}
+
+// l1 norms of the one-level analysis lowpass and highpass filters (libdwt's
+// normalisation), from the lifting steps applied to unit impulses of both
+// parities far from the line ends.
+void analysis_gains(const Coeffs<double>& cf, double* gl, double* gh) {
+  const int N = 64;
+  double sl = 0, sh = 0;
+  for (int p = N / 2; p < N / 2 + 2; ++p) {
+    double x[N] = {};
+    x[p] = 1.0;
+    for (int st = 0; st < cf.nsteps; ++st) {
+      const int par = (cf.first_parity + st) & 1;
+      for (int i = par; i < N; i += 2) {
+        const double l = i > 0 ? x[i - 1] : x[i + 1];
+        const double r = i + 1 < N ? x[i + 1] : x[i - 1];
+        x[i] += cf.c[st] * (l + r);
+      }
+    }
+    for (int i = 0; i < N; i += 2) sl += std::fabs(x[i] * cf.se);
+    for (int i = 1; i < N; i += 2) sh += std::fabs(x[i] * cf.so);
+  }
+  *gl = sl;
+  *gh = sh;
+}
+
+}  // namespace dwt

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -22,6 +23,7 @@
 
 #include "dwt_common.h"
 #include "kernels.h"
+#include "batch.h"
 #include "../../include/libdwt.h"
 
 namespace dwt {
@@ -168,6 +170,8 @@ struct Scratch {
   long long sync_cnt = 0;
   unsigned* err_host = nullptr;
   bool pyr_used = false;
+  unsigned long long* trace = nullptr;  // per-item timeline of the last pyramid launch
+  long long trace_n = 0, trace_items = 0;
   hipStream_t stream = nullptr;
   // side stream for the level-0 detail copies, which touch memory disjoint
   // from what the other levels read and write, so they overlap those levels
@@ -218,6 +222,9 @@ struct Scratch {
     sync_cnt = 0;
     if (err_host) hipHostFree(err_host);
     err_host = nullptr;
+    if (trace) hipFree(trace);
+    trace = nullptr;
+    trace_n = trace_items = 0;
   }
   static void free_plan(PyrPlan* pl) {
     if (pl->dev) hipFree(pl->dev);
@@ -246,7 +253,7 @@ static Scratch& scratch_for(hipStream_t st) {
 // ---- optional per-launch timing (HIP events on the launch stream) ---------------
 // Kinds: 0 forward level kernel, 1 inverse level kernel, 2 detail copy,
 // 4 forward block kernel, 5 inverse block kernel, 6 forward pyramid,
-// 7 inverse pyramid.  Level is the level index the
+// 7 inverse pyramid, 8 / 9 forward / inverse batched level launch.  Level is the level index the
 // launch belongs to (the first level it does for a block kernel).
 struct ProfRec {
   int kind, level;
@@ -303,6 +310,8 @@ static bool aligned_pair_host(const void* p, long long pitch) {
 // Schedule of the levels after level 0: 0 = one launch per level plus the
 // block kernel for the small ones (default), 1 = one pyramid launch.
 static int g_schedule = 0;
+static int g_trace_items = 0;  // > 0: pyramid launches record a per-item timeline
+static int g_pyr_grid_cap = 0;  // > 0: at most this many workgroups per pyramid launch
 static constexpr size_t kMaxPlans = 8;
 
 static unsigned long long spin_limit_ticks() {
@@ -393,9 +402,28 @@ static void run_pyr(bool fwd, std::vector<PyrTask>& tasks, int S, bool vec, cons
   sy.deps = reinterpret_cast<const int2*>(static_cast<char*>(plan->dev) + plan->deps_off);
   sy.ncnt = ncnt;
   sy.spin_limit = spin_limit_ticks();
+  sy.trace = nullptr;
+  if (g_trace_items > 0) {
+    if (!sc.trace || sc.trace_n < (long long)nitems) {
+      if (sc.trace) {
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipFree(sc.trace));
+        sc.trace = nullptr;
+      }
+      const long long n = std::max<long long>(nitems, g_trace_items);
+      if (hipMalloc(&sc.trace, sizeof(unsigned long long) * 4 * (size_t)n) != hipSuccess) {
+        (void)hipGetLastError();
+        sc.trace = nullptr;
+        fail(DWT_GPU_EMEMORY, "allocating the pyramid trace failed");
+      }
+      sc.trace_n = n;
+    }
+    sy.trace = sc.trace;
+    sc.trace_items = nitems;
+  }
   sc.pyr_used = true;
   LAUNCH(launch_pyr<T>(fwd, vec, S, static_cast<const PyrTask*>(plan->dev), ntasks, nitems, sy, cf,
-                       0, st));
+                       g_pyr_grid_cap, st));
 }
 
 // Reads and clears the pyramid error flag of sc (synchronises st); true when a
@@ -934,6 +962,277 @@ static void run_zero(void* p, int k0, int k1, int stride) {
   HIPCHK(hipStreamSynchronize(st));
 }
 
+// ---- batched transforms ------------------------------------------------------------
+// A plan holds everything a batch needs on the device: the per-level item
+// tables and block maps, the images for the max |x| pass, the gain table and
+// the per-image ping-pong scratch (LL of odd levels in B1, of even levels >= 2
+// in B2).  Executing a plan only launches kernels on the stream, so it can be
+// captured in a HIP graph and replayed.
+static int g_batch_rows[32] = {};  // segment rows per level (0: 8)
+
+struct BatchPlan {
+  bool fwd = true;
+  int wavelet = 0, storage = 0, n = 0, Lmax = 0;
+  std::vector<dwt_gpu_image> desc;
+  struct Level {
+    int level, items_off, nblocks, blocks_off, S;
+  };
+  std::vector<Level> levels;  // in execution order
+  int nimg_blocks = 0;
+  size_t off_items = 0, off_bmap = 0, off_imgs = 0, off_imap = 0, off_gain = 0, off_amax = 0;
+  void* dev = nullptr;
+  void* pinned = nullptr;
+  void* scratch = nullptr;
+  size_t bytes = 0, scratch_bytes = 0;
+  ~BatchPlan() {
+    if (dev) hipFree(dev);
+    if (pinned) hipHostFree(pinned);
+    if (scratch) hipFree(scratch);
+  }
+};
+
+static size_t storage_bytes(int storage) { return storage == kStoreF32 ? 4 : 2; }
+
+static BatchPlan* batch_plan_create(int wavelet, int direction, int storage,
+                                    dwt_gpu_image* images, int n, hipStream_t st) {
+  init_once();
+  if (wavelet != DWT_GPU_CDF97 && wavelet != DWT_GPU_CDF53) fail(DWT_GPU_EINVAL, "bad wavelet");
+  if (direction != DWT_GPU_FORWARD && direction != DWT_GPU_INVERSE) fail(DWT_GPU_EINVAL, "bad direction");
+  if (storage != kStoreF32 && storage != kStoreF16 && storage != kStoreBF16)
+    fail(DWT_GPU_EINVAL, "bad coefficient storage");
+  if (n <= 0 || !images) fail(DWT_GPU_EINVAL, "empty batch");
+  const bool fwd = direction == DWT_GPU_FORWARD;
+  const size_t cs = storage_bytes(storage);
+  const Coeffs<float> cf = make_coeffs<float>(wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53, fwd);
+  const int core = 2 * 64 - 2 * cf.nsteps;
+  std::unique_ptr<BatchPlan> pl(new BatchPlan());
+  pl->fwd = fwd;
+  pl->wavelet = wavelet;
+  pl->storage = storage;
+  pl->n = n;
+  // validate, clamp levels, size the scratch
+  std::vector<size_t> b1_off(n), b2_off(n);
+  std::vector<long long> b1p(n), b2p(n);
+  size_t scratch = 0;
+  for (int i = 0; i < n; ++i) {
+    dwt_gpu_image& im = images[i];
+    const int W = im.width, H = im.height;
+    if (W < 2 || H < 2) fail(DWT_GPU_EINVAL, "image %d: width and height must be at least 2", i);
+    if (!im.src || !im.dst) fail(DWT_GPU_EINVAL, "image %d: null buffer", i);
+    if (im.src == im.dst) fail(DWT_GPU_EINVAL, "image %d: batches are out of place (src == dst)", i);
+    const int jl = level_limit(W, H, 0);
+    int J = im.levels;
+    if (fwd) {
+      if (J < 0 || J > jl) J = jl;
+      im.levels = J;
+    } else if (J < 1 || J > jl) {
+      fail(DWT_GPU_EINVAL, "image %d: %d levels outside 1..%d", i, J, jl);
+    }
+    if (J < 1) fail(DWT_GPU_EINVAL, "image %d: no level to transform", i);
+    if (storage != kStoreF32 && !im.scales) fail(DWT_GPU_EINVAL, "image %d: 16-bit storage needs scales", i);
+    const long long ip = fwd ? im.src_pitch : im.dst_pitch;  // FP32 image side
+    const long long cp = fwd ? im.dst_pitch : im.src_pitch;  // coefficient side
+    const void* ipt = fwd ? im.src : (const void*)im.dst;
+    if (ip < 4LL * W || ip % 4 != 0 || (reinterpret_cast<uintptr_t>(ipt) & 3) != 0)
+      fail(DWT_GPU_EINVAL, "image %d: the FP32 image needs 4-byte alignment and pitch", i);
+    if (cp < (long long)cs * W || cp % (long long)cs != 0)
+      fail(DWT_GPU_EINVAL, "image %d: bad coefficient pitch", i);
+    pl->Lmax = std::max(pl->Lmax, J);
+    b1p[i] = (long long)pad_pitch(ceil_div_pow2(W, 1));
+    b2p[i] = (long long)pad_pitch(ceil_div_pow2(W, 2));
+    b1_off[i] = scratch;
+    scratch += (4 * (size_t)b1p[i] * ceil_div_pow2(H, 1) + 255) & ~size_t(255);
+    b2_off[i] = scratch;
+    if (J >= 3) scratch += (4 * (size_t)b2p[i] * ceil_div_pow2(H, 2) + 255) & ~size_t(255);
+  }
+  pl->desc.assign(images, images + n);
+  pl->scratch_bytes = scratch;
+  if (hipMalloc(&pl->scratch, std::max<size_t>(scratch, 256)) != hipSuccess) {
+    (void)hipGetLastError();
+    pl->scratch = nullptr;
+    fail(DWT_GPU_EMEMORY, "batch scratch of %zu bytes", scratch);
+  }
+  char* sb = static_cast<char*>(pl->scratch);
+  auto llbuf = [&](int i, int j) -> std::pair<float*, long long> {  // LL_j, j >= 1
+    return (j & 1) ? std::make_pair((float*)(sb + b1_off[i]), b1p[i])
+                   : std::make_pair((float*)(sb + b2_off[i]), b2p[i]);
+  };
+  // per-level item tables in execution order
+  std::vector<BatchItem> items;
+  std::vector<int> bmap;
+  for (int step = 0; step < pl->Lmax; ++step) {
+    const int j = fwd ? step : pl->Lmax - 1 - step;
+    BatchPlan::Level lv{j, (int)items.size(), 0, (int)bmap.size(), g_batch_rows[j] ? g_batch_rows[j] : 8};
+    int blocks = 0;
+    for (int i = 0; i < n; ++i) {
+      const dwt_gpu_image& im = images[i];
+      const int J = im.levels;
+      if (j >= J) continue;
+      BatchItem e{};
+      const long long ip = (fwd ? im.src_pitch : im.dst_pitch) / 4;
+      const long long cp = (fwd ? im.dst_pitch : im.src_pitch) / (long long)cs;
+      void* coef = fwd ? im.dst : const_cast<void*>(im.src);
+      e.W = ceil_div_pow2(im.width, j);
+      e.H = ceil_div_pow2(im.height, j);
+      e.level = j;
+      e.levels = J;
+      e.scale = im.scales;
+      e.det = coef;
+      e.dp = cp;
+      if (fwd) {
+        if (j == 0) {
+          e.src = im.src;
+          e.sp = ip;
+        } else {
+          e.src = llbuf(i, j).first;
+          e.sp = llbuf(i, j).second;
+        }
+        if (j == J - 1) {
+          e.out = coef;
+          e.op = cp;
+          e.flags = kItemStorageLL;
+        } else {
+          e.out = llbuf(i, j + 1).first;
+          e.op = llbuf(i, j + 1).second;
+        }
+      } else {
+        if (j == J - 1) {
+          e.src = coef;
+          e.sp = cp;
+          e.flags = kItemStorageLL;
+        } else {
+          e.src = llbuf(i, j + 1).first;
+          e.sp = llbuf(i, j + 1).second;
+        }
+        if (j == 0) {
+          e.out = im.dst;
+          e.op = ip;
+        } else {
+          e.out = llbuf(i, j).first;
+          e.op = llbuf(i, j).second;
+        }
+      }
+      // vector access on the FP32 rows: scratch rows always allow it, the
+      // image's only when pair-aligned
+      const bool img_vec = (reinterpret_cast<uintptr_t>(fwd ? im.src : (const void*)im.dst) & 7) == 0 &&
+                           ip % 2 == 0;
+      if (j > 0 || img_vec) e.flags |= kItemVec;
+      e.nstrips = (e.W + core - 1) / core;
+      e.ngroups = (e.nstrips + 3) / 4;
+      e.nseg = (e.H + lv.S - 1) / lv.S;
+      e.block0 = blocks;
+      const int nb = e.ngroups * e.nseg;
+      for (int b = 0; b < nb; ++b) bmap.push_back((int)(items.size() - lv.items_off));
+      blocks += nb;
+      items.push_back(e);
+    }
+    lv.nblocks = blocks;
+    pl->levels.push_back(lv);
+  }
+  // max |x| pass (forward, 16-bit storage)
+  std::vector<BatchImage> imgs;
+  std::vector<int> imap;
+  std::vector<float> gain(96 + 33, 1.0f);
+  if (fwd && storage != kStoreF32) {
+    for (int i = 0; i < n; ++i) {
+      const dwt_gpu_image& im = images[i];
+      BatchImage b{im.src, im.src_pitch / 4, im.scales, im.width, im.height, im.levels, (int)imap.size()};
+      const long long px = (long long)im.width * im.height;
+      const long long nb = (px + 256LL * kMaxAbsPerThread - 1) / (256LL * kMaxAbsPerThread);
+      for (long long k = 0; k < nb; ++k) imap.push_back(i);
+      imgs.push_back(b);
+    }
+    double gl = 0, gh = 0;
+    analysis_gains(make_coeffs<double>(wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53, true), &gl, &gh);
+    for (int j = 0; j < 32; ++j) {
+      const double in = std::pow(gl * gl, j);  // bound of LL_j relative to max |x|
+      gain[3 * j + 0] = (float)(in * gh * gl);
+      gain[3 * j + 1] = (float)(in * gl * gh);
+      gain[3 * j + 2] = (float)(in * gh * gh);
+    }
+    for (int j = 0; j <= 32; ++j) gain[96 + j] = (float)std::pow(gl * gl, j);
+  }
+  pl->nimg_blocks = (int)imap.size();
+  // device table: items | block map | images | image map | gains | amax
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  pl->off_items = 0;
+  pl->off_bmap = al(sizeof(BatchItem) * items.size());
+  pl->off_imgs = pl->off_bmap + al(sizeof(int) * bmap.size());
+  pl->off_imap = pl->off_imgs + al(sizeof(BatchImage) * imgs.size());
+  pl->off_gain = pl->off_imap + al(sizeof(int) * imap.size());
+  pl->off_amax = pl->off_gain + al(sizeof(float) * gain.size());
+  pl->bytes = pl->off_amax + al(sizeof(unsigned) * (size_t)n);
+  if (hipHostMalloc(&pl->pinned, pl->bytes, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(&pl->dev, pl->bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    fail(DWT_GPU_EMEMORY, "batch tables of %zu bytes", pl->bytes);
+  }
+  char* hp = static_cast<char*>(pl->pinned);
+  std::memset(hp, 0, pl->bytes);
+  std::memcpy(hp + pl->off_items, items.data(), sizeof(BatchItem) * items.size());
+  std::memcpy(hp + pl->off_bmap, bmap.data(), sizeof(int) * bmap.size());
+  if (!imgs.empty()) std::memcpy(hp + pl->off_imgs, imgs.data(), sizeof(BatchImage) * imgs.size());
+  if (!imap.empty()) std::memcpy(hp + pl->off_imap, imap.data(), sizeof(int) * imap.size());
+  std::memcpy(hp + pl->off_gain, gain.data(), sizeof(float) * gain.size());
+  HIPCHK(hipMemcpyAsync(pl->dev, pl->pinned, pl->bytes, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // a plan may then run on any stream
+  return pl.release();
+}
+
+static void batch_plan_execute(const BatchPlan& pl, hipStream_t st) {
+  const Coeffs<float> cf = make_coeffs<float>(pl.wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53, pl.fwd);
+  char* d = static_cast<char*>(pl.dev);
+  if (pl.fwd && pl.storage != kStoreF32) {
+    unsigned* amax = reinterpret_cast<unsigned*>(d + pl.off_amax);
+    const BatchImage* imgs = reinterpret_cast<const BatchImage*>(d + pl.off_imgs);
+    LAUNCH(launch_fill_u32(amax, 0u, pl.n, st);
+           launch_batch_maxabs(imgs, reinterpret_cast<const int*>(d + pl.off_imap), pl.nimg_blocks, amax, st);
+           launch_batch_scales(imgs, pl.n, amax, reinterpret_cast<const float*>(d + pl.off_gain), st));
+  }
+  for (const BatchPlan::Level& lv : pl.levels) {
+    ProfScope ps(pl.fwd ? 8 : 9, lv.level, st);
+    LAUNCH(launch_batch_level(pl.fwd, pl.storage, lv.S, lv.nblocks,
+                              reinterpret_cast<const BatchItem*>(d + pl.off_items) + lv.items_off,
+                              reinterpret_cast<const int*>(d + pl.off_bmap) + lv.blocks_off, cf, st));
+  }
+}
+
+// dwt_gpu_batch_2d keeps the plans of its most recent batches.
+static std::mutex g_batch_mu;
+static std::vector<BatchPlan*>* g_batch_cache = new std::vector<BatchPlan*>();
+static constexpr size_t kBatchCache = 4;
+
+static BatchPlan* batch_plan_cached(int wavelet, int direction, int storage, dwt_gpu_image* images,
+                                    int n, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  for (size_t i = 0; i < g_batch_cache->size(); ++i) {
+    BatchPlan* p = (*g_batch_cache)[i];
+    if (p->wavelet != wavelet || p->fwd != (direction == DWT_GPU_FORWARD) || p->storage != storage ||
+        p->n != n)
+      continue;
+    bool same = true;
+    for (int k = 0; k < n && same; ++k) {
+      dwt_gpu_image a = images[k];
+      const dwt_gpu_image& b = p->desc[k];
+      if (p->fwd && (a.levels < 0 || a.levels > b.levels)) a.levels = b.levels;  // clamped alike
+      same = std::memcmp(&a, &b, sizeof a) == 0;
+    }
+    if (!same) continue;
+    for (int k = 0; k < n; ++k) images[k].levels = p->desc[k].levels;
+    g_batch_cache->erase(g_batch_cache->begin() + (long)i);
+    g_batch_cache->insert(g_batch_cache->begin(), p);
+    return p;
+  }
+  BatchPlan* p = batch_plan_create(wavelet, direction, storage, images, n, st);
+  g_batch_cache->insert(g_batch_cache->begin(), p);
+  if (g_batch_cache->size() > kBatchCache) {
+    HIPCHK(hipDeviceSynchronize());  // the evicted plan may still be in use on some stream
+    delete g_batch_cache->back();
+    g_batch_cache->pop_back();
+  }
+  return p;
+}
+
 [[noreturn]] static void abort_with(const char* fn) {
   std::fprintf(stderr, "libdwt (MI355X): %s: %s\n", fn, g_err.c_str());
   std::fflush(stderr);
@@ -1149,6 +1448,39 @@ int dwt_gpu_schedule_set(int mode) {
   return DWT_GPU_OK;
 }
 
+int dwt_gpu_pyr_grid_cap_set(int max_workgroups) {
+  g_pyr_grid_cap = max_workgroups > 0 ? max_workgroups : 0;
+  return DWT_GPU_OK;
+}
+
+int dwt_gpu_clock_rate_khz(void) {
+  try {
+    return (int)(spin_limit_ticks() / 2000ULL);
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_pyr_trace_enable(int max_items) {
+  g_trace_items = max_items > 0 ? max_items : 0;
+  return DWT_GPU_OK;
+}
+
+long long dwt_gpu_pyr_trace_read(void* stream, unsigned long long* out, long long max_items) {
+  try {
+    hipStream_t st = (hipStream_t)stream;
+    Scratch& sc = scratch_for(st);
+    HIPCHK(hipStreamSynchronize(st));
+    if (!sc.trace || !out) return 0;
+    const long long n = std::min(max_items, sc.trace_items);
+    if (n > 0)
+      HIPCHK(hipMemcpy(out, sc.trace, sizeof(unsigned long long) * 4 * (size_t)n, hipMemcpyDeviceToHost));
+    return n;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
 int dwt_gpu_device_check(void* stream) {
   try {
     g_err.clear();
@@ -1159,6 +1491,58 @@ int dwt_gpu_device_check(void* stream) {
   } catch (const Failure& f) {
     return f.code;
   }
+}
+
+int dwt_gpu_batch_plan_create(int wavelet, int direction, int storage, dwt_gpu_image* images,
+                              int n, void* stream, void** plan) {
+  try {
+    g_err.clear();
+    if (!plan) fail(DWT_GPU_EINVAL, "null plan pointer");
+    *plan = batch_plan_create(wavelet, direction, storage, images, n, (hipStream_t)stream);
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_batch_plan_execute(void* plan, void* stream) {
+  try {
+    g_err.clear();
+    if (!plan) fail(DWT_GPU_EINVAL, "null plan");
+    batch_plan_execute(*static_cast<BatchPlan*>(plan), (hipStream_t)stream);
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_batch_plan_destroy(void* plan) {
+  if (plan) {
+    hipDeviceSynchronize();
+    delete static_cast<BatchPlan*>(plan);
+  }
+  return DWT_GPU_OK;
+}
+
+int dwt_gpu_batch_2d(int wavelet, int direction, int storage, dwt_gpu_image* images, int n,
+                     void* stream) {
+  try {
+    g_err.clear();
+    BatchPlan* p = batch_plan_cached(wavelet, direction, storage, images, n, (hipStream_t)stream);
+    batch_plan_execute(*p, (hipStream_t)stream);
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_batch_rows_set(int level, int seg_rows) {
+  if (level < 0 || level >= 32 || !(seg_rows == 0 || seg_rows == 8 || seg_rows == 16 || seg_rows == 32)) {
+    g_err = "dwt_gpu_batch_rows_set: level 0..31, rows 0, 8, 16 or 32";
+    return DWT_GPU_EINVAL;
+  }
+  g_batch_rows[level] = seg_rows;
+  return DWT_GPU_OK;
 }
 
 int dwt_gpu_tail_set(long long max_region_bytes) {

@@ -80,6 +80,8 @@ struct PyrTask {
 // zero before a launch (the launch leaves them zero); deps[ncnt] producer
 // segment range of each (task, segment); err set by a wait that ran longer
 // than spin_limit ticks of the constant-rate clock.
+// trace (optional, may be null): per item {start, dependencies met, done}
+// ticks of the constant-rate clock and the workgroup index.
 struct PyrSync {
   unsigned* ctrl;
   unsigned* cnt;
@@ -87,6 +89,7 @@ struct PyrSync {
   unsigned* err;
   int ncnt;
   unsigned long long spin_limit;
+  unsigned long long* trace;
 };
 // Assign items and counters to tasks (in table order); returns the item count.
 int pyr_layout(PyrTask* tasks, int ntasks, int nsteps);

@@ -105,6 +105,12 @@ __device__ __forceinline__ int pyr_next(const PyrSync& sy, int nitems, int* s_it
   return item < nitems ? item : -1;
 }
 
+// Optional per-item timeline (wave 0 of the workgroup writes it).
+__device__ __forceinline__ void pyr_trace(const PyrSync& sy, int item, int slot) {
+  if (sy.trace != nullptr && wave_index() == 0 && (threadIdx.x & (kWave - 1)) == 0)
+    sy.trace[4 * (long long)item + slot] = slot == 3 ? (unsigned long long)blockIdx.x : wall_clock64();
+}
+
 __device__ __forceinline__ void pyr_done(const PyrSync& sy, int counter) {
   __builtin_amdgcn_s_waitcnt(0);  // this wavefront's write-through stores are complete
   __syncthreads();                // ... and every other wavefront's
@@ -142,6 +148,8 @@ pyr_fwd_kernel(const PyrTask* __restrict__ tasks, int ntasks, int nitems, PyrSyn
   for (int iter = 0; iter < nitems; ++iter) {
     const int item = pyr_next(sy, nitems, &s_item);
     if (item < 0) break;
+    pyr_trace(sy, item, 0);
+    pyr_trace(sy, item, 3);
     const PyrTask& tk = tasks[pyr_find_task(tasks, ntasks, item)];
     const int local = item - tk.item0;
     const int s = local / tk.ngroups, g = local - s * tk.ngroups;
@@ -152,13 +160,15 @@ pyr_fwd_kernel(const PyrTask* __restrict__ tasks, int ntasks, int nitems, PyrSyn
       }
       __syncthreads();
     }
+    pyr_trace(sy, item, 1);
     const int strip = g * kWavesPerBlock + wave;
     if (strip < tk.nstrips)
       fwd_tile<T, NS, -SS>(strip, s, SS, tk.W, tk.H,
                            GlobalRows<T, VEC>{static_cast<const T*>(tk.src), tk.sp, tk.H},
-                           CoherentStore<T>{static_cast<T*>(tk.out), tk.op}, static_cast<T*>(tk.det),
-                           tk.dp, cf);
+                           CoherentStore<T>{static_cast<T*>(tk.out), tk.op},
+                           GlobalDet<T>{static_cast<T*>(tk.det), tk.dp}, cf);
     pyr_done(sy, tk.cnt0 + s);
+    pyr_trace(sy, item, 2);
   }
   pyr_finish(sy, &s_flag);
 }
@@ -175,6 +185,8 @@ pyr_inv_kernel(const PyrTask* __restrict__ tasks, int ntasks, int nitems, PyrSyn
   for (int iter = 0; iter < nitems; ++iter) {
     const int item = pyr_next(sy, nitems, &s_item);
     if (item < 0) break;
+    pyr_trace(sy, item, 0);
+    pyr_trace(sy, item, 3);
     const PyrTask& tk = tasks[pyr_find_task(tasks, ntasks, item)];
     const int local = item - tk.item0;
     const int s = local / tk.ngroups, g = local - s * tk.ngroups;
@@ -185,13 +197,15 @@ pyr_inv_kernel(const PyrTask* __restrict__ tasks, int ntasks, int nitems, PyrSyn
       }
       __syncthreads();
     }
+    pyr_trace(sy, item, 1);
     const int strip = g * kWavesPerBlock + wave;
     if (strip < tk.nstrips)
       inv_tile<T, NS, -SS>(strip, s, SS, tk.W, tk.H,
                            GlobalLoad<T>{static_cast<const T*>(tk.src), tk.sp},
-                           static_cast<const T*>(tk.det), tk.dp,
+                           GlobalDet<T>{static_cast<T*>(tk.det), tk.dp},
                            CoherentOut<T, VEC>{static_cast<T*>(tk.out), tk.op}, cf);
     pyr_done(sy, tk.cnt0 + s);
+    pyr_trace(sy, item, 2);
   }
   pyr_finish(sy, &s_flag);
 }

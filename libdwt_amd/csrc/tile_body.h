@@ -217,6 +217,10 @@ struct ColInv {
   }
 };
 
+// Detail subbands, by their Mallat position: HL top right (horizontal
+// highpass), LH bottom left, HH bottom right.
+constexpr int kHL = 0, kLH = 1, kHH = 2;
+
 // ---- level bodies -----------------------------------------------------------
 // One wavefront computes one tile: the core columns of strip `strip` and the
 // rows of segment `seg` (S rows).  Where rows come from and where the LL
@@ -229,11 +233,12 @@ struct ColInv {
 //   the mirrored column indices, `interior` says the wave's whole window lies
 //   inside the row.
 // StoreLL(m, k, v): LL coefficient (row m, column k) of the level output.
-template <typename T, int NS, int U, bool INTERIOR, class LoadRow, class StoreLL>
+// Det.store(band, r, c, v) / Det.load(band, r, c): a detail coefficient at row
+//   r, column c of the level's Mallat layout (band: kHL, kLH or kHH).
+template <typename T, int NS, int U, bool INTERIOR, class LoadRow, class StoreLL, class Det>
 __device__ __forceinline__ void fwd_tile_impl(int strip, int seg, int S, int W, int H,
                                               const LoadRow& load_row, const StoreLL& store_ll,
-                                              T* __restrict__ det, long long dp,
-                                              const Coeffs<T>& cf) {
+                                              const Det& det, const Coeffs<T>& cf) {
   constexpr int HALO = NS;
   constexpr int CORE = 2 * kWave - 2 * HALO;
   constexpr int LAG = NS / 2;
@@ -263,11 +268,11 @@ __device__ __forceinline__ void fwd_tile_impl(int strip, int seg, int S, int W, 
     if (m >= m0 && m < m1) {
       if (st_a) {
         store_ll(m, k, La);
-        if (m < Hh) st_global(det + (long long)(Hd + m) * dp + k, Ha);
+        if (m < Hh) det.store(kLH, Hd + m, k, Ha);
       }
       if (st_b) {
-        st_global(det + (long long)m * dp + Wd + k, Lb);
-        if (m < Hh) st_global(det + (long long)(Hd + m) * dp + Wd + k, Hb);
+        det.store(kHL, m, Wd + k, Lb);
+        if (m < Hh) det.store(kHH, Hd + m, Wd + k, Hb);
       }
     }
   };
@@ -306,28 +311,27 @@ __device__ __forceinline__ void fwd_tile_impl(int strip, int seg, int S, int W, 
 // vectorisable loads) or touches an end (mirrored columns).  The test is the
 // same for every lane of the wavefront; readfirstlane makes it a scalar branch
 // between two separately compiled bodies.
-template <typename T, int NS, int U, class LoadRow, class StoreLL>
+template <typename T, int NS, int U, class LoadRow, class StoreLL, class Det>
 __device__ __forceinline__ void fwd_tile(int strip, int seg, int S, int W, int H,
                                          const LoadRow& load_row, const StoreLL& store_ll,
-                                         T* __restrict__ det, long long dp, const Coeffs<T>& cf) {
+                                         const Det& det, const Coeffs<T>& cf) {
   constexpr int HALO = NS;
   const int x0 = strip * (2 * kWave - 2 * HALO);
   const int interior = __builtin_amdgcn_readfirstlane((x0 - HALO >= 0) && (x0 - HALO + 2 * kWave <= W));
   if (interior)
-    fwd_tile_impl<T, NS, U, true>(strip, seg, S, W, H, load_row, store_ll, det, dp, cf);
+    fwd_tile_impl<T, NS, U, true>(strip, seg, S, W, H, load_row, store_ll, det, cf);
   else
-    fwd_tile_impl<T, NS, U, false>(strip, seg, S, W, H, load_row, store_ll, det, dp, cf);
+    fwd_tile_impl<T, NS, U, false>(strip, seg, S, W, H, load_row, store_ll, det, cf);
 }
 
 // Inverse tile: output columns of strip `strip`, rows of segment `seg` of the
 // W x H fine region.  LoadLL(m, k): LL coefficient; details come from det at
 // their Mallat positions.  StoreOut(r, Xe, e, o, st_e, st_o): output samples
 // (Xe, Xe + 1) of row r.
-template <typename T, int NS, int U, class LoadLL, class StoreOut>
+template <typename T, int NS, int U, class LoadLL, class Det, class StoreOut>
 __device__ __forceinline__ void inv_tile(int strip, int seg, int S, int W, int H,
-                                         const LoadLL& load_ll, const T* __restrict__ det,
-                                         long long dp, const StoreOut& store_out,
-                                         const Coeffs<T>& cf) {
+                                         const LoadLL& load_ll, const Det& det,
+                                         const StoreOut& store_out, const Coeffs<T>& cf) {
   constexpr int HALO = NS;
   constexpr int CORE = 2 * kWave - 2 * HALO;
   constexpr int LAG = NS / 2;
@@ -348,12 +352,11 @@ __device__ __forceinline__ void inv_tile(int strip, int seg, int S, int W, int H
     const int Yr = mirror_near(Y, H);
     const int m = Yr >> 1;
     if (Yr & 1) {
-      const Gbl<T>* r = gload_ptr(det) + (long long)(Hc + m) * dp;
-      e = r[ke];
-      o = r[Wc + ko];
+      e = det.load(kLH, Hc + m, ke);
+      o = det.load(kHH, Hc + m, Wc + ko);
     } else {
       e = load_ll(m, ke);
-      o = gload_ptr(det)[(long long)m * dp + Wc + ko];
+      o = det.load(kHL, m, Wc + ko);
     }
   };
 
@@ -461,6 +464,17 @@ struct LdsLoad {
   const Lds<T>* p;
   int pitch;
   __device__ __forceinline__ T operator()(int m, int k) const { return p[m * pitch + k]; }
+};
+template <typename T>
+struct GlobalDet {  // detail subbands in the storage type, global
+  T* __restrict__ p;
+  long long pitch;
+  __device__ __forceinline__ void store(int, int r, int c, T v) const {
+    st_global(p + (long long)r * pitch + c, v);
+  }
+  __device__ __forceinline__ T load(int, int r, int c) const {
+    return gload_ptr(static_cast<const T*>(p))[(long long)r * pitch + c];
+  }
 };
 template <typename T, bool VEC>
 struct GlobalOut {  // two adjacent output samples, global

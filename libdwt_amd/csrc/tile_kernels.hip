@@ -19,7 +19,7 @@ fwd_level_kernel(const T* __restrict__ src, long long sp, int W, int H, T* __res
   const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
   fwd_tile<T, NS, U>(strip, s, S, W, H, GlobalRows<T, VEC>{src, sp, H}, GlobalStore<T>{ll, llp},
-                     det, dp, cf);
+                     GlobalDet<T>{det, dp}, cf);
 }
 
 template <typename T, int NS, bool VEC, int U>
@@ -31,8 +31,8 @@ inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ 
   const int g = b / nseg, s = b - g * nseg;
   const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
-  inv_tile<T, NS, U>(strip, s, S, W, H, GlobalLoad<T>{ll, llp}, det, dp,
-                     GlobalOut<T, VEC>{dst, dstp}, cf);
+  inv_tile<T, NS, U>(strip, s, S, W, H, GlobalLoad<T>{ll, llp},
+                     GlobalDet<T>{const_cast<T*>(det), dp}, GlobalOut<T, VEC>{dst, dstp}, cf);
 }
 
 // ---- block kernels (one workgroup, a whole small pyramid in LDS) --------------

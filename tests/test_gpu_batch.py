@@ -1,0 +1,163 @@
+"""GPU: batched transforms of images of uneven sizes (one launch per level for
+the whole batch), through the C ABI.
+
+Bars:
+  * FP32 storage: every image's coefficients and reconstruction bit-identical
+    to the oracle's single-image transform (the reference's _s arithmetic);
+  * FP16 / BF16 storage: every stored coefficient equals the FP32 one scaled
+    by its subband's power of two and rounded to nearest (so within half an
+    ulp of the 16-bit format, relative 2^-11 / 2^-8, or the FP16 subnormal
+    spacing), the scales are powers of two with |c * s| <= 2^14, and the
+    reconstruction from 16-bit coefficients stays within the tolerance below.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import libdwt_amd
+from libdwt_amd import workloads
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+SIZES = [(64, 64), (65, 33), (2, 2), (130, 71), (300, 257), (1031, 64), (64, 1031), (513, 511),
+         (2, 9), (127, 3)]
+# reconstruction error bound relative to max |x| for 16-bit storage (measured
+# 2e-4 for FP16 and 2e-3 for BF16 on these images, J up to 9)
+RECON_TOL = {"f16": 2e-3, "bf16": 2e-2}
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    assert libdwt_amd.lib().dwt_gpu_init() == 0
+    return torch.device("cuda:0")
+
+
+def images(seed=5):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, (W, H) in enumerate(SIZES):
+        if k % 2:
+            out.append(workloads.sinusoid_image(W, H, seed + k))
+        else:
+            out.append(rng.standard_normal((H, W)).astype(np.float32))
+    return out
+
+
+@pytest.mark.parametrize("wavelet", ["cdf97", "cdf53"])
+def test_batch_f32_matches_single_image_oracle(wavelet):
+    from oracle.oracle import oracle
+    o = oracle()
+    imgs = images()
+    say(f"batch f32 {wavelet}: {len(imgs)} images")
+    ts = [torch.from_numpy(x).cuda() for x in imgs]
+    levels = [-1, 3, -1, 5, -1, 2, -1, 9, 1, 1]
+    coefs, _, done = libdwt_amd.batch_forward(ts, wavelet, "f32", levels)
+    torch.cuda.synchronize()
+    for x, c, lv, j in zip(imgs, coefs, levels, done):
+        ref, jr = o.fwd2(x, wavelet, lv)
+        assert j == jr
+        np.testing.assert_array_equal(c.cpu().numpy(), ref, err_msg=f"forward {x.shape} J={j}")
+    refs = [torch.from_numpy(o.fwd2(x, wavelet, lv)[0]).cuda() for x, lv in zip(imgs, levels)]
+    outs = libdwt_amd.batch_inverse(refs, done, wavelet, "f32")
+    torch.cuda.synchronize()
+    for x, r, out, j in zip(imgs, refs, outs, done):
+        want = o.inv2(r.cpu().numpy(), wavelet, j)
+        np.testing.assert_array_equal(out.cpu().numpy(), want, err_msg=f"inverse {x.shape} J={j}")
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("wavelet", ["cdf97", "cdf53"])
+def test_batch_16bit_storage(wavelet, storage):
+    imgs = images(9)
+    say(f"batch {storage} {wavelet}")
+    ts = [torch.from_numpy(x).cuda() for x in imgs]
+    c32, _, done = libdwt_amd.batch_forward(ts, wavelet, "f32", -1)
+    c16, scales, done16 = libdwt_amd.batch_forward(ts, wavelet, storage, -1)
+    torch.cuda.synchronize()
+    assert done16 == done
+    mant = 11 if storage == "f16" else 8
+    for x, a, b, sc, j in zip(imgs, c32, c16, scales, done):
+        H, W = x.shape
+        s = sc.cpu().numpy().astype(np.float64)
+        assert s.shape == (3 * j + 1,)
+        assert np.all(np.isfinite(s)) and np.all(s > 0)
+        assert np.all(np.frexp(s)[0] == 0.5), "scales are powers of two"
+        smap = np.full((H, W), np.nan)
+        for lev in range(j):
+            w, h = -(-W // 2 ** lev), -(-H // 2 ** lev)
+            wd, hd = -(-w // 2), -(-h // 2)
+            smap[:hd, wd:w] = s[3 * lev + 0]
+            smap[hd:h, :wd] = s[3 * lev + 1]
+            smap[hd:h, wd:w] = s[3 * lev + 2]
+        wl, hl = -(-W // 2 ** j), -(-H // 2 ** j)
+        smap[:hl, :wl] = s[3 * j]
+        assert not np.isnan(smap).any()
+        c = a.cpu().numpy().astype(np.float64)
+        q = b.float().cpu().numpy().astype(np.float64)
+        assert np.abs(c * smap).max() <= 2.0 ** 14 * (1 + 2.0 ** -mant)
+        # round to nearest: error of the scaled value <= half an ulp (or half
+        # the FP16 subnormal spacing)
+        err = np.abs(q - c * smap)
+        ulp_half = np.maximum(np.abs(c * smap) * 2.0 ** -mant, 2.0 ** -25 if storage == "f16" else 0)
+        assert np.all(err <= ulp_half * 1.0000001 + 1e-45), (x.shape, float((err / ulp_half).max()))
+    outs = libdwt_amd.batch_inverse(c16, done, wavelet, storage, scales)
+    torch.cuda.synchronize()
+    for x, out in zip(imgs, outs):
+        e = float(np.abs(out.cpu().numpy() - x).max()) / float(np.abs(x).max())
+        say(f"  {x.shape} relative reconstruction error {e:.2e}")
+        assert e <= RECON_TOL[storage]
+
+
+def test_batch_plan_replays_and_graph_capture():
+    """A plan executed repeatedly, and captured in a HIP graph and replayed
+    three times (eager work and syncs in between), gives the eager result
+    every time."""
+    import ctypes
+    imgs = images(3)[:6]
+    ts = [torch.from_numpy(x).cuda() for x in imgs]
+    outs = [torch.empty_like(t) for t in ts]
+    arr = libdwt_amd.batch_descriptors(ts, outs, -1)
+    L = libdwt_amd.lib()
+    s = torch.cuda.Stream()
+    plan = ctypes.c_void_p()
+    assert L.dwt_gpu_batch_plan_create(0, 0, 0, arr, len(ts), s.cuda_stream, ctypes.byref(plan)) == 0
+    try:
+        assert L.dwt_gpu_batch_plan_execute(plan, s.cuda_stream) == 0
+        s.synchronize()
+        eager = [o.clone() for o in outs]
+        for o in outs:
+            o.zero_()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            assert L.dwt_gpu_batch_plan_execute(plan, s.cuda_stream) == 0
+        for rep in range(3):
+            for o in outs:
+                o.fill_(float(rep + 1))
+            torch.cuda.synchronize()
+            g.replay()
+            torch.cuda.synchronize()
+            for a, b in zip(outs, eager):
+                assert torch.equal(a, b), f"replay {rep}"
+    finally:
+        L.dwt_gpu_batch_plan_destroy(plan)
+
+
+def test_batch_rejects_bad_descriptors():
+    t = torch.zeros((8, 8), device="cuda")
+    L = libdwt_amd.lib()
+    arr = libdwt_amd.batch_descriptors([t], [t], -1)  # in place
+    assert L.dwt_gpu_batch_2d(0, 0, 0, arr, 1, None) == -1
+    u = torch.zeros((8, 8), device="cuda")
+    arr = libdwt_amd.batch_descriptors([t], [u], -1)
+    assert L.dwt_gpu_batch_2d(0, 0, 1, arr, 1, None) == -1  # 16-bit without scales
+    one = torch.zeros((1, 8), device="cuda")
+    arr = libdwt_amd.batch_descriptors([one], [u[:1]], -1)
+    assert L.dwt_gpu_batch_2d(0, 0, 0, arr, 1, None) == -1  # height 1
