@@ -58,9 +58,13 @@ def parse():
                     help="levels after level 0: 1 one pyramid launch, 0 one launch per level")
     ap.add_argument("--cold-steps", type=int, default=10,
                     help="steps timed one at a time after a 512 MiB cache scrub")
-    ap.add_argument("--workload", default="image", choices=("image", "batch"),
+    ap.add_argument("--workload", default="image", choices=("image", "batch", "serve"),
                     help="image: one 4096x4096 image (the headline metric); batch: 256 "
-                         "images of uneven sizes, batched launches vs one launch per image")
+                         "images of uneven sizes, batched launches vs one launch per image; "
+                         "serve: Poisson request stream through the serving loop")
+    ap.add_argument("--rate", type=float, default=8000.0,
+                    help="serve: offered request rate (requests/s)")
+    ap.add_argument("--requests", type=int, default=4000, help="serve: requests per run")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -269,6 +273,78 @@ def bench_batch(args, torch, libdwt_amd, workloads):
     print(json.dumps(out), flush=True)
 
 
+def bench_serve(args, torch, libdwt_amd, workloads):
+    """Serving: a Poisson stream of requests at --rate requests/s; each request
+    is one float32 image with sides log-uniform in [128, 4096] (64 distinct
+    resident images, seed 7), CDF 9/7 or 5/3 (50/50), forward or inverse
+    (50/50; an inverse request reads the image's samples as coefficients),
+    J uniform in [1, min(8, j_limit)].  The same request sequence runs through
+    the continuous-batching server and through a server that takes one
+    request at a time on one stream (one transform per request); latency is
+    submission to observed completion."""
+    import math
+    from libdwt_amd import serving
+    rng = np.random.default_rng(7)
+    sizes = workloads.uneven_sizes(64, 128, 4096, seed=7)
+    dev = torch.device("cuda:0")
+    pool = [torch.from_numpy(workloads.sinusoid_image(w, h, seed=2000 + i)).to(dev)
+            for i, (w, h) in enumerate(sizes)]
+    n = args.requests
+    reqs = []
+    for _ in range(n):
+        k = int(rng.integers(len(pool)))
+        w, h = sizes[k]
+        jmax = min(8, workloads.ceil_log2(min(w, h)))
+        reqs.append((k, "cdf97" if rng.random() < 0.5 else "cdf53", bool(rng.random() < 0.5),
+                     int(rng.integers(1, jmax + 1))))
+    gaps = rng.exponential(1.0 / args.rate, n)
+    outs = [torch.empty_like(pool[k]) for k, _, _, _ in reqs]
+    area = sum(pool[k].numel() for k, _, _, _ in reqs)
+    torch.cuda.synchronize()
+
+    def run(max_batch, streams):
+        with serving.Server(16 << 30, max_batch=max_batch, streams=streams) as s:
+            # warm-up: one request of each kind
+            for k, wv, fwd, j in reqs[:8]:
+                s.submit(wv, fwd, pool[k], torch.empty_like(pool[k]), j)
+            s.drain()
+            s.poll(1 << 16)
+            b0, _ = s.stats()
+            t0 = time.perf_counter()
+            due = t0
+            for (k, wv, fwd, j), out, gap in zip(reqs, outs, gaps):
+                due += gap
+                while time.perf_counter() < due:
+                    pass
+                s.submit(wv, fwd, pool[k], out, j)
+            s.drain()
+            t1 = time.perf_counter()
+            lat = np.array([x[1] for x in s.poll(1 << 20)])
+            b1, _ = s.stats()
+        lat.sort()
+        return {"p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+                "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+                "mean_ms": round(1e3 * float(lat.mean()), 3),
+                "Mpix_s": round(area / (t1 - t0) / 1e6, 1),
+                "seconds": round(t1 - t0, 4), "batches": b1 - b0,
+                "offered_s": round(float(gaps.sum()), 4)}
+
+    batched = run(256, 4)
+    plain = run(1, 1)
+    out = {
+        "metric": "serving latency p50/p99 and Mpix/s, Poisson stream, uneven images 128..4096",
+        "value": batched["Mpix_s"], "unit": "Mpix/s",
+        "n_gpus": 1, "steps": n, "warmup": 8, "ms_per_step": batched["mean_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (64 resident sinusoid+noise images, seeds 2000..2063; request mix seed 7)",
+        "config": {"workload": "serve", "rate_per_s": args.rate, "requests": n,
+                   "max_batch": 256, "streams": 4, "budget_gib": 16},
+        "batched": batched, "one_request_at_a_time": plain,
+        "p99_ratio_plain_over_batched": round(plain["p99_ms"] / batched["p99_ms"], 2),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -296,6 +372,9 @@ def main():
     libdwt_amd.schedule_set(args.schedule)
     if args.workload == "batch":
         bench_batch(args, torch, libdwt_amd, workloads)
+        return
+    if args.workload == "serve":
+        bench_serve(args, torch, libdwt_amd, workloads)
         return
 
     W, H, J, wv = args.width, args.height, args.levels, args.wavelet

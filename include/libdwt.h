@@ -227,6 +227,34 @@ int dwt_gpu_batch_plan_destroy(void *plan);
  * level, for plans created afterwards. */
 int dwt_gpu_batch_rows_set(int level, int seg_rows);
 
+/* ---- serving loop -----------------------------------------------------------
+ * A server takes transform requests (one image each, buffers resident in
+ * device memory, semantics of one image of dwt_gpu_batch_2d) from any thread
+ * at any time and runs them with continuous batching: a scheduler thread
+ * owns `streams` streams; whenever one is free it admits waiting requests in
+ * arrival order while their scratch fits the HBM budget (allocated once, at
+ * creation) and the batch holds at most max_batch requests, and launches
+ * them as one batch on that stream.  Requests arriving meanwhile join the
+ * next batch, so batches on different streams overlap.  Each result equals
+ * the result of the request alone.  poll() returns completed requests (id,
+ * latency from submission to observed completion in seconds, status). */
+typedef struct dwt_gpu_request {
+	int wavelet;    /* DWT_GPU_CDF97 / DWT_GPU_CDF53 */
+	int direction;  /* DWT_GPU_FORWARD / DWT_GPU_INVERSE */
+	int storage;    /* DWT_GPU_STORE_* of the coefficients */
+	int reserved;
+	dwt_gpu_image image;
+} dwt_gpu_request;
+int dwt_gpu_server_create(long long hbm_budget_bytes, int max_batch, int streams, void **server);
+int dwt_gpu_server_submit(void *server, const dwt_gpu_request *request, long long *id);
+/* Up to max completed requests; returns their count. */
+int dwt_gpu_server_poll(void *server, long long *ids, double *latency_s, int *status, int max);
+/* Wait until every submitted request has completed. */
+int dwt_gpu_server_drain(void *server);
+int dwt_gpu_server_stats(void *server, long long *batches, long long *requests);
+/* Waits for the requests in flight, then frees the server. */
+int dwt_gpu_server_destroy(void *server);
+
 /* Bytes of device scratch the calling thread currently holds for `stream`. *//* Bytes of device scratch the calling thread currently holds for `stream`. */
 long long dwt_gpu_scratch_bytes(void *stream);
 /* Release the calling thread's scratch for `stream` (synchronises it). */

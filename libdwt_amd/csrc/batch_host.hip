@@ -1,0 +1,364 @@
+// Host side of the batched transforms: plans (tables built once, executed
+// any number of times) and their C entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "batch.h"
+#include "batch_tables.h"
+#include "host_common.h"
+#include "kernels.h"
+
+namespace dwt {
+
+// ---- batched transforms ------------------------------------------------------------
+// A batch is described by its tables: per level (in execution order) the
+// items of every image at that level and the block -> item map, plus, for
+// 16-bit storage, the images of the max |x| pass and the gain bounds.  Each
+// image also needs its ping-pong scratch (LL of odd levels in B1, of even
+// levels >= 2 in B2).  A plan builds the tables once and keeps them on the
+// device; executing it only launches kernels, so it can be captured in a HIP
+// graph and replayed.  The serving loop builds tables per admitted batch.
+static int g_batch_rows[32] = {};  // segment rows per level (0: 8)
+
+static size_t storage_bytes(int storage) { return storage == kStoreF32 ? 4 : 2; }
+
+// Validate one image of a batch and clamp its levels (forward).
+void batch_check_image(int i, dwt_gpu_image& im, bool fwd, int storage) {
+  const int W = im.width, H = im.height;
+  const size_t cs = storage_bytes(storage);
+  if (W < 2 || H < 2) fail(DWT_GPU_EINVAL, "image %d: width and height must be at least 2", i);
+  if (!im.src || !im.dst) fail(DWT_GPU_EINVAL, "image %d: null buffer", i);
+  if (im.src == im.dst) fail(DWT_GPU_EINVAL, "image %d: batches are out of place (src == dst)", i);
+  const int jl = level_limit(W, H, 0);
+  if (fwd) {
+    if (im.levels < 0 || im.levels > jl) im.levels = jl;
+  } else if (im.levels < 1 || im.levels > jl) {
+    fail(DWT_GPU_EINVAL, "image %d: %d levels outside 1..%d", i, im.levels, jl);
+  }
+  if (im.levels < 1) fail(DWT_GPU_EINVAL, "image %d: no level to transform", i);
+  if (storage != kStoreF32 && !im.scales) fail(DWT_GPU_EINVAL, "image %d: 16-bit storage needs scales", i);
+  const long long ip = fwd ? im.src_pitch : im.dst_pitch;  // FP32 image side
+  const long long cp = fwd ? im.dst_pitch : im.src_pitch;  // coefficient side
+  const void* ipt = fwd ? im.src : (const void*)im.dst;
+  if (ip < 4LL * W || ip % 4 != 0 || (reinterpret_cast<uintptr_t>(ipt) & 3) != 0)
+    fail(DWT_GPU_EINVAL, "image %d: the FP32 image needs 4-byte alignment and pitch", i);
+  if (cp < (long long)cs * W || cp % (long long)cs != 0)
+    fail(DWT_GPU_EINVAL, "image %d: bad coefficient pitch", i);
+}
+
+void batch_check_kind(int wavelet, int direction, int storage) {
+  if (wavelet != DWT_GPU_CDF97 && wavelet != DWT_GPU_CDF53) fail(DWT_GPU_EINVAL, "bad wavelet");
+  if (direction != DWT_GPU_FORWARD && direction != DWT_GPU_INVERSE) fail(DWT_GPU_EINVAL, "bad direction");
+  if (storage != kStoreF32 && storage != kStoreF16 && storage != kStoreBF16)
+    fail(DWT_GPU_EINVAL, "bad coefficient storage");
+}
+
+// Scratch bytes of one image (B1 then B2, 256-byte aligned).
+size_t image_scratch_bytes(const dwt_gpu_image& im) {
+  const size_t b1 = (4 * pad_pitch(ceil_div_pow2(im.width, 1)) * ceil_div_pow2(im.height, 1) + 255) & ~size_t(255);
+  const size_t b2 = im.levels >= 3 ? (4 * pad_pitch(ceil_div_pow2(im.width, 2)) * ceil_div_pow2(im.height, 2) + 255) & ~size_t(255) : 0;
+  return b1 + b2;
+}
+
+void BatchTables::build(int wavelet_, bool fwd_, int storage_, const dwt_gpu_image* images, int n_,
+                      char* const* scratch) {
+  fwd = fwd_;
+  wavelet = wavelet_;
+  storage = storage_;
+  n = n_;
+  const size_t cs = storage_bytes(storage);
+  const Coeffs<float> cf = make_coeffs<float>(wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53, fwd);
+  const int core = 2 * 64 - 2 * cf.nsteps;
+  Lmax = 0;
+  for (int i = 0; i < n; ++i) Lmax = std::max(Lmax, images[i].levels);
+  items.clear();
+  bmap.clear();
+  levels.clear();
+  auto llbuf = [&](int i, int j) -> std::pair<float*, long long> {  // LL_j, j >= 1
+    const dwt_gpu_image& im = images[i];
+    const long long p1 = (long long)pad_pitch(ceil_div_pow2(im.width, 1));
+    const long long p2 = (long long)pad_pitch(ceil_div_pow2(im.width, 2));
+    const size_t b1 = (4 * (size_t)p1 * ceil_div_pow2(im.height, 1) + 255) & ~size_t(255);
+    return (j & 1) ? std::make_pair((float*)scratch[i], p1)
+                   : std::make_pair((float*)(scratch[i] + b1), p2);
+  };
+  for (int step = 0; step < Lmax; ++step) {
+    const int j = fwd ? step : Lmax - 1 - step;
+    BatchLevel lv{j, (int)items.size(), 0, (int)bmap.size(), g_batch_rows[j] ? g_batch_rows[j] : 8};
+    int blocks = 0;
+    for (int i = 0; i < n; ++i) {
+      const dwt_gpu_image& im = images[i];
+      const int J = im.levels;
+      if (j >= J) continue;
+      BatchItem e{};
+      const long long ip = (fwd ? im.src_pitch : im.dst_pitch) / 4;
+      const long long cp = (fwd ? im.dst_pitch : im.src_pitch) / (long long)cs;
+      void* coef = fwd ? im.dst : const_cast<void*>(im.src);
+      e.W = ceil_div_pow2(im.width, j);
+      e.H = ceil_div_pow2(im.height, j);
+      e.level = j;
+      e.levels = J;
+      e.scale = im.scales;
+      e.det = coef;
+      e.dp = cp;
+      if (fwd) {
+        if (j == 0) {
+          e.src = im.src;
+          e.sp = ip;
+        } else {
+          e.src = llbuf(i, j).first;
+          e.sp = llbuf(i, j).second;
+        }
+        if (j == J - 1) {
+          e.out = coef;
+          e.op = cp;
+          e.flags = kItemStorageLL;
+        } else {
+          e.out = llbuf(i, j + 1).first;
+          e.op = llbuf(i, j + 1).second;
+        }
+      } else {
+        if (j == J - 1) {
+          e.src = coef;
+          e.sp = cp;
+          e.flags = kItemStorageLL;
+        } else {
+          e.src = llbuf(i, j + 1).first;
+          e.sp = llbuf(i, j + 1).second;
+        }
+        if (j == 0) {
+          e.out = im.dst;
+          e.op = ip;
+        } else {
+          e.out = llbuf(i, j).first;
+          e.op = llbuf(i, j).second;
+        }
+      }
+      // vector access on the FP32 rows: scratch rows always allow it, the
+      // image's only when pair-aligned
+      const bool img_vec = (reinterpret_cast<uintptr_t>(fwd ? im.src : (const void*)im.dst) & 7) == 0 &&
+                           ip % 2 == 0;
+      if (j > 0 || img_vec) e.flags |= kItemVec;
+      e.nstrips = (e.W + core - 1) / core;
+      e.ngroups = (e.nstrips + 3) / 4;
+      e.nseg = (e.H + lv.S - 1) / lv.S;
+      e.block0 = blocks;
+      const int nb = e.ngroups * e.nseg;
+      const int idx = (int)(items.size() - lv.items_off);
+      bmap.insert(bmap.end(), nb, idx);
+      blocks += nb;
+      items.push_back(e);
+    }
+    lv.nblocks = blocks;
+    levels.push_back(lv);
+  }
+  imgs.clear();
+  imap.clear();
+  gain.assign(96 + 33, 1.0f);
+  if (fwd && storage != kStoreF32) {
+    for (int i = 0; i < n; ++i) {
+      const dwt_gpu_image& im = images[i];
+      BatchImage b{im.src, im.src_pitch / 4, im.scales, im.width, im.height, im.levels, (int)imap.size()};
+      const long long px = (long long)im.width * im.height;
+      const long long nb = (px + 256LL * kMaxAbsPerThread - 1) / (256LL * kMaxAbsPerThread);
+      imap.insert(imap.end(), (size_t)nb, i);
+      imgs.push_back(b);
+    }
+    double gl = 0, gh = 0;
+    analysis_gains(make_coeffs<double>(wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53, true), &gl, &gh);
+    for (int j = 0; j < 32; ++j) {
+      const double in = std::pow(gl * gl, j);  // bound of LL_j relative to max |x|
+      gain[3 * j + 0] = (float)(in * gh * gl);
+      gain[3 * j + 1] = (float)(in * gl * gh);
+      gain[3 * j + 2] = (float)(in * gh * gh);
+    }
+    for (int j = 0; j <= 32; ++j) gain[96 + j] = (float)std::pow(gl * gl, j);
+  }
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  off_items = 0;
+  off_bmap = al(sizeof(BatchItem) * items.size());
+  off_imgs = off_bmap + al(sizeof(int) * bmap.size());
+  off_imap = off_imgs + al(sizeof(BatchImage) * imgs.size());
+  off_gain = off_imap + al(sizeof(int) * imap.size());
+  off_amax = off_gain + al(sizeof(float) * gain.size());
+  bytes = off_amax + al(sizeof(unsigned) * (size_t)n);
+}
+
+void BatchTables::pack(char* hp) const {
+  std::memcpy(hp + off_items, items.data(), sizeof(BatchItem) * items.size());
+  std::memcpy(hp + off_bmap, bmap.data(), sizeof(int) * bmap.size());
+  if (!imgs.empty()) std::memcpy(hp + off_imgs, imgs.data(), sizeof(BatchImage) * imgs.size());
+  if (!imap.empty()) std::memcpy(hp + off_imap, imap.data(), sizeof(int) * imap.size());
+  std::memcpy(hp + off_gain, gain.data(), sizeof(float) * gain.size());
+}
+
+void BatchTables::launch(char* dev, hipStream_t st) const {
+  const Coeffs<float> cf = make_coeffs<float>(wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53, fwd);
+  if (fwd && storage != kStoreF32) {
+    unsigned* amax = reinterpret_cast<unsigned*>(dev + off_amax);
+    const BatchImage* im = reinterpret_cast<const BatchImage*>(dev + off_imgs);
+    LAUNCH(launch_fill_u32(amax, 0u, n, st);
+           launch_batch_maxabs(im, reinterpret_cast<const int*>(dev + off_imap), (int)imap.size(), amax, st);
+           launch_batch_scales(im, n, amax, reinterpret_cast<const float*>(dev + off_gain), st));
+  }
+  for (const BatchLevel& lv : levels) {
+    ProfScope ps(fwd ? 8 : 9, lv.level, st);
+    LAUNCH(launch_batch_level(fwd, storage, lv.S, lv.nblocks,
+                              reinterpret_cast<const BatchItem*>(dev + off_items) + lv.items_off,
+                              reinterpret_cast<const int*>(dev + off_bmap) + lv.blocks_off, cf, st));
+  }
+}
+
+struct BatchPlan {
+  std::vector<dwt_gpu_image> desc;  // as given, levels clamped
+  BatchTables t;
+  void* dev = nullptr;
+  void* pinned = nullptr;
+  void* scratch = nullptr;
+  ~BatchPlan() {
+    if (dev) hipFree(dev);
+    if (pinned) hipHostFree(pinned);
+    if (scratch) hipFree(scratch);
+  }
+};
+
+static BatchPlan* batch_plan_create(int wavelet, int direction, int storage,
+                                    dwt_gpu_image* images, int n, hipStream_t st) {
+  init_once();
+  batch_check_kind(wavelet, direction, storage);
+  if (n <= 0 || !images) fail(DWT_GPU_EINVAL, "empty batch");
+  const bool fwd = direction == DWT_GPU_FORWARD;
+  std::unique_ptr<BatchPlan> pl(new BatchPlan());
+  std::vector<size_t> off(n);
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) {
+    batch_check_image(i, images[i], fwd, storage);
+    off[i] = total;
+    total += image_scratch_bytes(images[i]);
+  }
+  pl->desc.assign(images, images + n);
+  if (hipMalloc(&pl->scratch, std::max<size_t>(total, 256)) != hipSuccess) {
+    (void)hipGetLastError();
+    pl->scratch = nullptr;
+    fail(DWT_GPU_EMEMORY, "batch scratch of %zu bytes", total);
+  }
+  std::vector<char*> base(n);
+  for (int i = 0; i < n; ++i) base[i] = static_cast<char*>(pl->scratch) + off[i];
+  pl->t.build(wavelet, fwd, storage, images, n, base.data());
+  if (hipHostMalloc(&pl->pinned, pl->t.bytes, hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(&pl->dev, pl->t.bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    fail(DWT_GPU_EMEMORY, "batch tables of %zu bytes", pl->t.bytes);
+  }
+  std::memset(pl->pinned, 0, pl->t.bytes);
+  pl->t.pack(static_cast<char*>(pl->pinned));
+  HIPCHK(hipMemcpyAsync(pl->dev, pl->pinned, pl->t.bytes, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // a plan may then run on any stream
+  return pl.release();
+}
+
+static void batch_plan_execute(const BatchPlan& pl, hipStream_t st) {
+  pl.t.launch(static_cast<char*>(pl.dev), st);
+}
+
+// dwt_gpu_batch_2d keeps the plans of its most recent batches.
+static std::mutex g_batch_mu;
+static std::vector<BatchPlan*>* g_batch_cache = new std::vector<BatchPlan*>();
+static constexpr size_t kBatchCache = 4;
+
+static BatchPlan* batch_plan_cached(int wavelet, int direction, int storage, dwt_gpu_image* images,
+                                    int n, hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_batch_mu);
+  for (size_t i = 0; i < g_batch_cache->size(); ++i) {
+    BatchPlan* p = (*g_batch_cache)[i];
+    if (p->t.wavelet != wavelet || p->t.fwd != (direction == DWT_GPU_FORWARD) ||
+        p->t.storage != storage || p->t.n != n)
+      continue;
+    bool same = true;
+    for (int k = 0; k < n && same; ++k) {
+      dwt_gpu_image a = images[k];
+      const dwt_gpu_image& b = p->desc[k];
+      if (p->t.fwd && (a.levels < 0 || a.levels > b.levels)) a.levels = b.levels;  // clamped alike
+      same = std::memcmp(&a, &b, sizeof a) == 0;
+    }
+    if (!same) continue;
+    for (int k = 0; k < n; ++k) images[k].levels = p->desc[k].levels;
+    g_batch_cache->erase(g_batch_cache->begin() + (long)i);
+    g_batch_cache->insert(g_batch_cache->begin(), p);
+    return p;
+  }
+  BatchPlan* p = batch_plan_create(wavelet, direction, storage, images, n, st);
+  g_batch_cache->insert(g_batch_cache->begin(), p);
+  if (g_batch_cache->size() > kBatchCache) {
+    HIPCHK(hipDeviceSynchronize());  // the evicted plan may still be in use on some stream
+    delete g_batch_cache->back();
+    g_batch_cache->pop_back();
+  }
+  return p;
+}
+
+}  // namespace dwt
+
+using namespace dwt;
+
+extern "C" {
+
+int dwt_gpu_batch_plan_create(int wavelet, int direction, int storage, dwt_gpu_image* images,
+                              int n, void* stream, void** plan) {
+  try {
+    g_err.clear();
+    if (!plan) fail(DWT_GPU_EINVAL, "null plan pointer");
+    *plan = batch_plan_create(wavelet, direction, storage, images, n, (hipStream_t)stream);
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_batch_plan_execute(void* plan, void* stream) {
+  try {
+    g_err.clear();
+    if (!plan) fail(DWT_GPU_EINVAL, "null plan");
+    batch_plan_execute(*static_cast<BatchPlan*>(plan), (hipStream_t)stream);
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_batch_plan_destroy(void* plan) {
+  if (plan) {
+    hipDeviceSynchronize();
+    delete static_cast<BatchPlan*>(plan);
+  }
+  return DWT_GPU_OK;
+}
+
+int dwt_gpu_batch_2d(int wavelet, int direction, int storage, dwt_gpu_image* images, int n,
+                     void* stream) {
+  try {
+    g_err.clear();
+    BatchPlan* p = batch_plan_cached(wavelet, direction, storage, images, n, (hipStream_t)stream);
+    batch_plan_execute(*p, (hipStream_t)stream);
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_batch_rows_set(int level, int seg_rows) {
+  if (level < 0 || level >= 32 || !(seg_rows == 0 || seg_rows == 8 || seg_rows == 16 || seg_rows == 32)) {
+    g_err = "dwt_gpu_batch_rows_set: level 0..31, rows 0, 8, 16 or 32";
+    return DWT_GPU_EINVAL;
+  }
+  g_batch_rows[level] = seg_rows;
+  return DWT_GPU_OK;
+}
+
+}  // extern "C"

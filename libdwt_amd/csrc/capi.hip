@@ -23,8 +23,7 @@
 
 #include "dwt_common.h"
 #include "kernels.h"
-#include "batch.h"
-#include "../../include/libdwt.h"
+#include "host_common.h"
 
 namespace dwt {
 
@@ -103,13 +102,9 @@ Coeffs<double> make_coeffs<double>(Wavelet w, bool fwd) {
 }
 
 // ---- errors -------------------------------------------------------------------
-static thread_local std::string g_err;
+thread_local std::string g_err;
 
-struct Failure {
-  int code;
-};
-
-static void fail(int code, const char* fmt, ...) {
+void fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
   va_start(ap, fmt);
@@ -119,26 +114,10 @@ static void fail(int code, const char* fmt, ...) {
   throw Failure{code};
 }
 
-#define HIPCHK(expr)                                                                  \
-  do {                                                                                \
-    hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess) fail(DWT_GPU_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
-
-// hipGetLastError() returns the last error of *any* earlier HIP call of this
-// thread and keeps it until read, so a failure left behind by other code would
-// be reported as ours.  LAUNCH() discards such a stale error first and then
-// checks only what the enclosed launches returned.
-static void check_launch() {
+void check_launch() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) fail(DWT_GPU_EHIP, "kernel launch: %s", hipGetErrorString(e));
 }
-#define LAUNCH(...)            \
-  do {                         \
-    (void)hipGetLastError();   \
-    __VA_ARGS__;               \
-    check_launch();            \
-  } while (0)
 
 // ---- device scratch, one set per (host thread, stream, device) ----------------
 struct DevBuf {
@@ -251,45 +230,32 @@ static Scratch& scratch_for(hipStream_t st) {
 }
 
 // ---- optional per-launch timing (HIP events on the launch stream) ---------------
-// Kinds: 0 forward level kernel, 1 inverse level kernel, 2 detail copy,
-// 4 forward block kernel, 5 inverse block kernel, 6 forward pyramid,
-// 7 inverse pyramid, 8 / 9 forward / inverse batched level launch.  Level is the level index the
-// launch belongs to (the first level it does for a block kernel).
-struct ProfRec {
-  int kind, level;
-  hipEvent_t a, b;
-};
 static std::mutex g_prof_mu;
 static bool g_prof_on = false;
 static std::vector<ProfRec>* g_prof = new std::vector<ProfRec>();
 
-struct ProfScope {
-  bool on = false;
-  ProfRec r{};
-  hipStream_t st;
-  ProfScope(int kind, int level, hipStream_t s) : st(s) {
-    {
-      std::lock_guard<std::mutex> lk(g_prof_mu);
-      on = g_prof_on;
-    }
-    if (!on) return;
-    r.kind = kind;
-    r.level = level;
-    HIPCHK(hipEventCreate(&r.a));
-    HIPCHK(hipEventCreate(&r.b));
-    HIPCHK(hipEventRecord(r.a, st));
-  }
-  ~ProfScope() {
-    if (!on) return;
-    hipEventRecord(r.b, st);
+ProfScope::ProfScope(int kind, int level, hipStream_t s) : st(s) {
+  {
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    g_prof->push_back(r);
+    on = g_prof_on;
   }
-};
+  if (!on) return;
+  r.kind = kind;
+  r.level = level;
+  HIPCHK(hipEventCreate(&r.a));
+  HIPCHK(hipEventCreate(&r.b));
+  HIPCHK(hipEventRecord(r.a, st));
+}
+ProfScope::~ProfScope() {
+  if (!on) return;
+  hipEventRecord(r.b, st);
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof->push_back(r);
+}
 
 static std::once_flag g_init_once;
 static hipError_t g_init_status = hipSuccess;
-static void init_once() {
+void init_once() {
   std::call_once(g_init_once, [] {
     g_init_status = configure_line_kernels();
     if (g_init_status == hipSuccess) g_init_status = configure_tail_kernels();
@@ -298,7 +264,6 @@ static void init_once() {
     fail(DWT_GPU_EHIP, "raising the kernels' LDS limits: %s", hipGetErrorString(g_init_status));
 }
 
-static size_t pad_pitch(size_t w) { return (w + 63) & ~size_t(63); }
 
 // Two-element vector access is possible: pointer aligned to a pair, even pitch.
 template <typename T>
@@ -761,7 +726,7 @@ static int fast_levels(int W, int H, int J) {
   return jf;
 }
 
-static int level_limit(int Wo, int Ho, int d1) {
+int level_limit(int Wo, int Ho, int d1) {
   const int mn = Wo < Ho ? Wo : Ho, mx = Wo < Ho ? Ho : Wo;
   return ceil_log2(d1 ? mx : mn);  // src/libdwt.c:6324
 }
@@ -960,277 +925,6 @@ static void run_zero(void* p, int k0, int k1, int stride) {
   const int zy[4] = {0, 0, 0, 0};
   LAUNCH(launch_zero_pad<T>((char*)dp, 0, stride, k1, 1, zx, zy, st));
   HIPCHK(hipStreamSynchronize(st));
-}
-
-// ---- batched transforms ------------------------------------------------------------
-// A plan holds everything a batch needs on the device: the per-level item
-// tables and block maps, the images for the max |x| pass, the gain table and
-// the per-image ping-pong scratch (LL of odd levels in B1, of even levels >= 2
-// in B2).  Executing a plan only launches kernels on the stream, so it can be
-// captured in a HIP graph and replayed.
-static int g_batch_rows[32] = {};  // segment rows per level (0: 8)
-
-struct BatchPlan {
-  bool fwd = true;
-  int wavelet = 0, storage = 0, n = 0, Lmax = 0;
-  std::vector<dwt_gpu_image> desc;
-  struct Level {
-    int level, items_off, nblocks, blocks_off, S;
-  };
-  std::vector<Level> levels;  // in execution order
-  int nimg_blocks = 0;
-  size_t off_items = 0, off_bmap = 0, off_imgs = 0, off_imap = 0, off_gain = 0, off_amax = 0;
-  void* dev = nullptr;
-  void* pinned = nullptr;
-  void* scratch = nullptr;
-  size_t bytes = 0, scratch_bytes = 0;
-  ~BatchPlan() {
-    if (dev) hipFree(dev);
-    if (pinned) hipHostFree(pinned);
-    if (scratch) hipFree(scratch);
-  }
-};
-
-static size_t storage_bytes(int storage) { return storage == kStoreF32 ? 4 : 2; }
-
-static BatchPlan* batch_plan_create(int wavelet, int direction, int storage,
-                                    dwt_gpu_image* images, int n, hipStream_t st) {
-  init_once();
-  if (wavelet != DWT_GPU_CDF97 && wavelet != DWT_GPU_CDF53) fail(DWT_GPU_EINVAL, "bad wavelet");
-  if (direction != DWT_GPU_FORWARD && direction != DWT_GPU_INVERSE) fail(DWT_GPU_EINVAL, "bad direction");
-  if (storage != kStoreF32 && storage != kStoreF16 && storage != kStoreBF16)
-    fail(DWT_GPU_EINVAL, "bad coefficient storage");
-  if (n <= 0 || !images) fail(DWT_GPU_EINVAL, "empty batch");
-  const bool fwd = direction == DWT_GPU_FORWARD;
-  const size_t cs = storage_bytes(storage);
-  const Coeffs<float> cf = make_coeffs<float>(wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53, fwd);
-  const int core = 2 * 64 - 2 * cf.nsteps;
-  std::unique_ptr<BatchPlan> pl(new BatchPlan());
-  pl->fwd = fwd;
-  pl->wavelet = wavelet;
-  pl->storage = storage;
-  pl->n = n;
-  // validate, clamp levels, size the scratch
-  std::vector<size_t> b1_off(n), b2_off(n);
-  std::vector<long long> b1p(n), b2p(n);
-  size_t scratch = 0;
-  for (int i = 0; i < n; ++i) {
-    dwt_gpu_image& im = images[i];
-    const int W = im.width, H = im.height;
-    if (W < 2 || H < 2) fail(DWT_GPU_EINVAL, "image %d: width and height must be at least 2", i);
-    if (!im.src || !im.dst) fail(DWT_GPU_EINVAL, "image %d: null buffer", i);
-    if (im.src == im.dst) fail(DWT_GPU_EINVAL, "image %d: batches are out of place (src == dst)", i);
-    const int jl = level_limit(W, H, 0);
-    int J = im.levels;
-    if (fwd) {
-      if (J < 0 || J > jl) J = jl;
-      im.levels = J;
-    } else if (J < 1 || J > jl) {
-      fail(DWT_GPU_EINVAL, "image %d: %d levels outside 1..%d", i, J, jl);
-    }
-    if (J < 1) fail(DWT_GPU_EINVAL, "image %d: no level to transform", i);
-    if (storage != kStoreF32 && !im.scales) fail(DWT_GPU_EINVAL, "image %d: 16-bit storage needs scales", i);
-    const long long ip = fwd ? im.src_pitch : im.dst_pitch;  // FP32 image side
-    const long long cp = fwd ? im.dst_pitch : im.src_pitch;  // coefficient side
-    const void* ipt = fwd ? im.src : (const void*)im.dst;
-    if (ip < 4LL * W || ip % 4 != 0 || (reinterpret_cast<uintptr_t>(ipt) & 3) != 0)
-      fail(DWT_GPU_EINVAL, "image %d: the FP32 image needs 4-byte alignment and pitch", i);
-    if (cp < (long long)cs * W || cp % (long long)cs != 0)
-      fail(DWT_GPU_EINVAL, "image %d: bad coefficient pitch", i);
-    pl->Lmax = std::max(pl->Lmax, J);
-    b1p[i] = (long long)pad_pitch(ceil_div_pow2(W, 1));
-    b2p[i] = (long long)pad_pitch(ceil_div_pow2(W, 2));
-    b1_off[i] = scratch;
-    scratch += (4 * (size_t)b1p[i] * ceil_div_pow2(H, 1) + 255) & ~size_t(255);
-    b2_off[i] = scratch;
-    if (J >= 3) scratch += (4 * (size_t)b2p[i] * ceil_div_pow2(H, 2) + 255) & ~size_t(255);
-  }
-  pl->desc.assign(images, images + n);
-  pl->scratch_bytes = scratch;
-  if (hipMalloc(&pl->scratch, std::max<size_t>(scratch, 256)) != hipSuccess) {
-    (void)hipGetLastError();
-    pl->scratch = nullptr;
-    fail(DWT_GPU_EMEMORY, "batch scratch of %zu bytes", scratch);
-  }
-  char* sb = static_cast<char*>(pl->scratch);
-  auto llbuf = [&](int i, int j) -> std::pair<float*, long long> {  // LL_j, j >= 1
-    return (j & 1) ? std::make_pair((float*)(sb + b1_off[i]), b1p[i])
-                   : std::make_pair((float*)(sb + b2_off[i]), b2p[i]);
-  };
-  // per-level item tables in execution order
-  std::vector<BatchItem> items;
-  std::vector<int> bmap;
-  for (int step = 0; step < pl->Lmax; ++step) {
-    const int j = fwd ? step : pl->Lmax - 1 - step;
-    BatchPlan::Level lv{j, (int)items.size(), 0, (int)bmap.size(), g_batch_rows[j] ? g_batch_rows[j] : 8};
-    int blocks = 0;
-    for (int i = 0; i < n; ++i) {
-      const dwt_gpu_image& im = images[i];
-      const int J = im.levels;
-      if (j >= J) continue;
-      BatchItem e{};
-      const long long ip = (fwd ? im.src_pitch : im.dst_pitch) / 4;
-      const long long cp = (fwd ? im.dst_pitch : im.src_pitch) / (long long)cs;
-      void* coef = fwd ? im.dst : const_cast<void*>(im.src);
-      e.W = ceil_div_pow2(im.width, j);
-      e.H = ceil_div_pow2(im.height, j);
-      e.level = j;
-      e.levels = J;
-      e.scale = im.scales;
-      e.det = coef;
-      e.dp = cp;
-      if (fwd) {
-        if (j == 0) {
-          e.src = im.src;
-          e.sp = ip;
-        } else {
-          e.src = llbuf(i, j).first;
-          e.sp = llbuf(i, j).second;
-        }
-        if (j == J - 1) {
-          e.out = coef;
-          e.op = cp;
-          e.flags = kItemStorageLL;
-        } else {
-          e.out = llbuf(i, j + 1).first;
-          e.op = llbuf(i, j + 1).second;
-        }
-      } else {
-        if (j == J - 1) {
-          e.src = coef;
-          e.sp = cp;
-          e.flags = kItemStorageLL;
-        } else {
-          e.src = llbuf(i, j + 1).first;
-          e.sp = llbuf(i, j + 1).second;
-        }
-        if (j == 0) {
-          e.out = im.dst;
-          e.op = ip;
-        } else {
-          e.out = llbuf(i, j).first;
-          e.op = llbuf(i, j).second;
-        }
-      }
-      // vector access on the FP32 rows: scratch rows always allow it, the
-      // image's only when pair-aligned
-      const bool img_vec = (reinterpret_cast<uintptr_t>(fwd ? im.src : (const void*)im.dst) & 7) == 0 &&
-                           ip % 2 == 0;
-      if (j > 0 || img_vec) e.flags |= kItemVec;
-      e.nstrips = (e.W + core - 1) / core;
-      e.ngroups = (e.nstrips + 3) / 4;
-      e.nseg = (e.H + lv.S - 1) / lv.S;
-      e.block0 = blocks;
-      const int nb = e.ngroups * e.nseg;
-      for (int b = 0; b < nb; ++b) bmap.push_back((int)(items.size() - lv.items_off));
-      blocks += nb;
-      items.push_back(e);
-    }
-    lv.nblocks = blocks;
-    pl->levels.push_back(lv);
-  }
-  // max |x| pass (forward, 16-bit storage)
-  std::vector<BatchImage> imgs;
-  std::vector<int> imap;
-  std::vector<float> gain(96 + 33, 1.0f);
-  if (fwd && storage != kStoreF32) {
-    for (int i = 0; i < n; ++i) {
-      const dwt_gpu_image& im = images[i];
-      BatchImage b{im.src, im.src_pitch / 4, im.scales, im.width, im.height, im.levels, (int)imap.size()};
-      const long long px = (long long)im.width * im.height;
-      const long long nb = (px + 256LL * kMaxAbsPerThread - 1) / (256LL * kMaxAbsPerThread);
-      for (long long k = 0; k < nb; ++k) imap.push_back(i);
-      imgs.push_back(b);
-    }
-    double gl = 0, gh = 0;
-    analysis_gains(make_coeffs<double>(wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53, true), &gl, &gh);
-    for (int j = 0; j < 32; ++j) {
-      const double in = std::pow(gl * gl, j);  // bound of LL_j relative to max |x|
-      gain[3 * j + 0] = (float)(in * gh * gl);
-      gain[3 * j + 1] = (float)(in * gl * gh);
-      gain[3 * j + 2] = (float)(in * gh * gh);
-    }
-    for (int j = 0; j <= 32; ++j) gain[96 + j] = (float)std::pow(gl * gl, j);
-  }
-  pl->nimg_blocks = (int)imap.size();
-  // device table: items | block map | images | image map | gains | amax
-  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
-  pl->off_items = 0;
-  pl->off_bmap = al(sizeof(BatchItem) * items.size());
-  pl->off_imgs = pl->off_bmap + al(sizeof(int) * bmap.size());
-  pl->off_imap = pl->off_imgs + al(sizeof(BatchImage) * imgs.size());
-  pl->off_gain = pl->off_imap + al(sizeof(int) * imap.size());
-  pl->off_amax = pl->off_gain + al(sizeof(float) * gain.size());
-  pl->bytes = pl->off_amax + al(sizeof(unsigned) * (size_t)n);
-  if (hipHostMalloc(&pl->pinned, pl->bytes, hipHostMallocDefault) != hipSuccess ||
-      hipMalloc(&pl->dev, pl->bytes) != hipSuccess) {
-    (void)hipGetLastError();
-    fail(DWT_GPU_EMEMORY, "batch tables of %zu bytes", pl->bytes);
-  }
-  char* hp = static_cast<char*>(pl->pinned);
-  std::memset(hp, 0, pl->bytes);
-  std::memcpy(hp + pl->off_items, items.data(), sizeof(BatchItem) * items.size());
-  std::memcpy(hp + pl->off_bmap, bmap.data(), sizeof(int) * bmap.size());
-  if (!imgs.empty()) std::memcpy(hp + pl->off_imgs, imgs.data(), sizeof(BatchImage) * imgs.size());
-  if (!imap.empty()) std::memcpy(hp + pl->off_imap, imap.data(), sizeof(int) * imap.size());
-  std::memcpy(hp + pl->off_gain, gain.data(), sizeof(float) * gain.size());
-  HIPCHK(hipMemcpyAsync(pl->dev, pl->pinned, pl->bytes, hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));  // a plan may then run on any stream
-  return pl.release();
-}
-
-static void batch_plan_execute(const BatchPlan& pl, hipStream_t st) {
-  const Coeffs<float> cf = make_coeffs<float>(pl.wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53, pl.fwd);
-  char* d = static_cast<char*>(pl.dev);
-  if (pl.fwd && pl.storage != kStoreF32) {
-    unsigned* amax = reinterpret_cast<unsigned*>(d + pl.off_amax);
-    const BatchImage* imgs = reinterpret_cast<const BatchImage*>(d + pl.off_imgs);
-    LAUNCH(launch_fill_u32(amax, 0u, pl.n, st);
-           launch_batch_maxabs(imgs, reinterpret_cast<const int*>(d + pl.off_imap), pl.nimg_blocks, amax, st);
-           launch_batch_scales(imgs, pl.n, amax, reinterpret_cast<const float*>(d + pl.off_gain), st));
-  }
-  for (const BatchPlan::Level& lv : pl.levels) {
-    ProfScope ps(pl.fwd ? 8 : 9, lv.level, st);
-    LAUNCH(launch_batch_level(pl.fwd, pl.storage, lv.S, lv.nblocks,
-                              reinterpret_cast<const BatchItem*>(d + pl.off_items) + lv.items_off,
-                              reinterpret_cast<const int*>(d + pl.off_bmap) + lv.blocks_off, cf, st));
-  }
-}
-
-// dwt_gpu_batch_2d keeps the plans of its most recent batches.
-static std::mutex g_batch_mu;
-static std::vector<BatchPlan*>* g_batch_cache = new std::vector<BatchPlan*>();
-static constexpr size_t kBatchCache = 4;
-
-static BatchPlan* batch_plan_cached(int wavelet, int direction, int storage, dwt_gpu_image* images,
-                                    int n, hipStream_t st) {
-  std::lock_guard<std::mutex> lk(g_batch_mu);
-  for (size_t i = 0; i < g_batch_cache->size(); ++i) {
-    BatchPlan* p = (*g_batch_cache)[i];
-    if (p->wavelet != wavelet || p->fwd != (direction == DWT_GPU_FORWARD) || p->storage != storage ||
-        p->n != n)
-      continue;
-    bool same = true;
-    for (int k = 0; k < n && same; ++k) {
-      dwt_gpu_image a = images[k];
-      const dwt_gpu_image& b = p->desc[k];
-      if (p->fwd && (a.levels < 0 || a.levels > b.levels)) a.levels = b.levels;  // clamped alike
-      same = std::memcmp(&a, &b, sizeof a) == 0;
-    }
-    if (!same) continue;
-    for (int k = 0; k < n; ++k) images[k].levels = p->desc[k].levels;
-    g_batch_cache->erase(g_batch_cache->begin() + (long)i);
-    g_batch_cache->insert(g_batch_cache->begin(), p);
-    return p;
-  }
-  BatchPlan* p = batch_plan_create(wavelet, direction, storage, images, n, st);
-  g_batch_cache->insert(g_batch_cache->begin(), p);
-  if (g_batch_cache->size() > kBatchCache) {
-    HIPCHK(hipDeviceSynchronize());  // the evicted plan may still be in use on some stream
-    delete g_batch_cache->back();
-    g_batch_cache->pop_back();
-  }
-  return p;
 }
 
 [[noreturn]] static void abort_with(const char* fn) {
@@ -1491,58 +1185,6 @@ int dwt_gpu_device_check(void* stream) {
   } catch (const Failure& f) {
     return f.code;
   }
-}
-
-int dwt_gpu_batch_plan_create(int wavelet, int direction, int storage, dwt_gpu_image* images,
-                              int n, void* stream, void** plan) {
-  try {
-    g_err.clear();
-    if (!plan) fail(DWT_GPU_EINVAL, "null plan pointer");
-    *plan = batch_plan_create(wavelet, direction, storage, images, n, (hipStream_t)stream);
-    return DWT_GPU_OK;
-  } catch (const Failure& f) {
-    return f.code;
-  }
-}
-
-int dwt_gpu_batch_plan_execute(void* plan, void* stream) {
-  try {
-    g_err.clear();
-    if (!plan) fail(DWT_GPU_EINVAL, "null plan");
-    batch_plan_execute(*static_cast<BatchPlan*>(plan), (hipStream_t)stream);
-    return DWT_GPU_OK;
-  } catch (const Failure& f) {
-    return f.code;
-  }
-}
-
-int dwt_gpu_batch_plan_destroy(void* plan) {
-  if (plan) {
-    hipDeviceSynchronize();
-    delete static_cast<BatchPlan*>(plan);
-  }
-  return DWT_GPU_OK;
-}
-
-int dwt_gpu_batch_2d(int wavelet, int direction, int storage, dwt_gpu_image* images, int n,
-                     void* stream) {
-  try {
-    g_err.clear();
-    BatchPlan* p = batch_plan_cached(wavelet, direction, storage, images, n, (hipStream_t)stream);
-    batch_plan_execute(*p, (hipStream_t)stream);
-    return DWT_GPU_OK;
-  } catch (const Failure& f) {
-    return f.code;
-  }
-}
-
-int dwt_gpu_batch_rows_set(int level, int seg_rows) {
-  if (level < 0 || level >= 32 || !(seg_rows == 0 || seg_rows == 8 || seg_rows == 16 || seg_rows == 32)) {
-    g_err = "dwt_gpu_batch_rows_set: level 0..31, rows 0, 8, 16 or 32";
-    return DWT_GPU_EINVAL;
-  }
-  g_batch_rows[level] = seg_rows;
-  return DWT_GPU_OK;
 }
 
 int dwt_gpu_tail_set(long long max_region_bytes) {
