@@ -58,10 +58,11 @@ def parse():
                     help="levels after level 0: 1 one pyramid launch, 0 one launch per level")
     ap.add_argument("--cold-steps", type=int, default=10,
                     help="steps timed one at a time after a 512 MiB cache scrub")
-    ap.add_argument("--workload", default="image", choices=("image", "batch", "serve"),
+    ap.add_argument("--workload", default="image", choices=("image", "batch", "serve", "lines"),
                     help="image: one 4096x4096 image (the headline metric); batch: 256 "
                          "images of uneven sizes, batched launches vs one launch per image; "
-                         "serve: Poisson request stream through the serving loop")
+                         "serve: Poisson request stream through the serving loop; lines: "
+                         "16384 random walks of 8192 samples, full-depth 1D")
     ap.add_argument("--rate", type=float, default=8000.0,
                     help="serve: offered request rate (requests/s)")
     ap.add_argument("--requests", type=int, default=4000, help="serve: requests per run")
@@ -345,6 +346,88 @@ def bench_serve(args, torch, libdwt_amd, workloads):
     print(json.dumps(out), flush=True)
 
 
+def bench_lines(args, torch, libdwt_amd, workloads):
+    """16384 float32 random walks (cumulative sums of N(0,1), seed 11) of 8192
+    samples, full-depth forward then inverse (13 levels), stored contiguous
+    (signal = row) and interleaved (signal = column of an 8192 x 16384
+    matrix).  Algorithmic bytes: every level reads and writes its n samples."""
+    count, N = 16384, 8192
+    sig = workloads.random_walk(count, N, seed=11)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    sp = stream.cuda_stream
+    wv = args.wavelet
+    levels = workloads.ceil_log2(N)
+    n, alg = N, 0
+    while n >= 2:
+        alg += 2 * 4 * n
+        n = (n + 1) // 2
+    alg *= 2 * count  # forward + inverse, all signals
+    res = {}
+    for layout in ("contiguous", "interleaved"):
+        t = torch.from_numpy(sig if layout == "contiguous" else np.ascontiguousarray(sig.T)).to(dev)
+        if layout == "contiguous":
+            a = (t.data_ptr(), 4 * N, 4)
+        else:
+            a = (t.data_ptr(), 4, 4 * count)
+
+        def step():
+            j = libdwt_amd.lines_1d(wv, True, a[0], a[1], a[2], N, count, -1, stream=sp)
+            libdwt_amd.lines_1d(wv, False, a[0], a[1], a[2], N, count, j, stream=sp)
+
+        torch.cuda.synchronize()
+        for _ in range(max(1, args.warmup)):
+            step()
+        ms = _timed(torch, stream, step, max(1, args.steps))
+        back = t.cpu().numpy()
+        if layout == "interleaved":
+            back = back.T
+        err = float(np.abs(back - sig).max())
+        res[layout] = {"ms_per_step": round(ms, 4),
+                       "Msamples_s": round(count * N / (ms * 1e-3) / 1e6, 1),
+                       "GB_s": round(alg / (ms * 1e-3) / 1e9, 1),
+                       "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "max_abs_roundtrip_err": err}
+        del t
+    out = {
+        "metric": f"Msamples/s, 16384 x 8192 random walks, full-depth 1D {wv} fwd+inv",
+        "value": res["interleaved"]["Msamples_s"], "unit": "Msamples/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["interleaved"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (random walks, seed 11)",
+        "config": {"workload": "lines", "signals": count, "length": N, "levels": levels,
+                   "wavelet": wv},
+        "layouts": res,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_lines_baseline(sig, wv, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_lines_baseline(sig: np.ndarray, wavelet: str, budget_s: float) -> dict:
+    """The reference's per-line functions in the caller loop (full depth,
+    forward + inverse) on a bounded sample of the signals, one host thread
+    (the per-line calls are single-threaded)."""
+    from oracle import oracle as orc
+    lib = orc.Reference() if orc.have_ref() else orc.oracle()
+    kind = "reference" if orc.have_ref() else "port"
+    sample = np.ascontiguousarray(sig[:64])
+    N = sample.shape[1]
+    best = float("inf")
+    t_end = time.perf_counter() + budget_s
+    reps = 0
+    while reps < 2 or (time.perf_counter() < t_end and reps < 50):
+        t0 = time.perf_counter()
+        for r in sample:
+            c = lib.f_full_1d(r, wavelet)
+            lib.i_full_1d(c, wavelet)
+        best = min(best, time.perf_counter() - t0)
+        reps += 1
+    return {"value": round(sample.size / best / 1e6, 3), "unit": "Msamples/s", "cores": 1,
+            "kind": kind, "sample": f"64 of the 16384 signals, contiguous, min of {reps} runs"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -375,6 +458,9 @@ def main():
         return
     if args.workload == "serve":
         bench_serve(args, torch, libdwt_amd, workloads)
+        return
+    if args.workload == "lines":
+        bench_lines(args, torch, libdwt_amd, workloads)
         return
 
     W, H, J, wv = args.width, args.height, args.levels, args.wavelet

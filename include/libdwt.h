@@ -183,6 +183,21 @@ int dwt_gpu_clock_rate_khz(void);
 int dwt_gpu_pyr_grid_cap_set(int max_workgroups);
 long long dwt_gpu_pyr_trace_read(void *stream, unsigned long long *out, long long max_items);
 
+/* ---- full-depth 1D transforms of many signals ---------------------------------
+ * count signals of `length` samples, in place: signal k, sample i at
+ * ptr + k * signal_stride + i * sample_stride (bytes, device-accessible
+ * memory).  The result is that of the caller loop over
+ * dwt_cdf97_f_ex_stride_s (src/libdwt.h:219-226) with L and H written back
+ * to [0, ceil(n/2)) and [ceil(n/2), n) of each signal for n = length,
+ * ceil(length/2), ... while n >= 2; the inverse undoes it.  *levels: forward,
+ * requested (< 0 or too large: full depth) and written back; inverse, the
+ * levels to undo.  Asynchronous on `stream`.  Fast paths: contiguous samples
+ * (length <= 16384 FP32 / 8192 FP64), and adjacent signals (signal_stride =
+ * element size: the columns of a matrix); any other layout runs one launch
+ * per level. */
+int dwt_gpu_1d_batch(int wavelet, int direction, int precision, void *ptr, long long signal_stride,
+                     long long sample_stride, int length, int count, int *levels, void *stream);
+
 /* ---- batches of images of uneven sizes ------------------------------------
  * One call transforms many packed single-channel images, out of place, with
  * one launch per level for the whole batch.  Each image: src -> dst;

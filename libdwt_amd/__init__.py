@@ -121,6 +121,9 @@ def lib() -> C.CDLL:
         L.dwt_gpu_batch_plan_destroy.restype = i
         L.dwt_gpu_batch_rows_set.argtypes = [i, i]
         L.dwt_gpu_batch_rows_set.restype = i
+        L.dwt_gpu_1d_batch.argtypes = [i, i, i, vp, C.c_longlong, C.c_longlong, i, i,
+                                       C.POINTER(i), vp]
+        L.dwt_gpu_1d_batch.restype = i
         _lib = L
     return _lib
 
@@ -196,6 +199,18 @@ def pyr_trace(stream: int = 0, max_items: int = 1 << 20):
     out = rows.astype(np.float64)
     out[:, :3] = (rows[:, :3] - t0) * (1000.0 / khz)
     return out
+
+
+def lines_1d(wavelet: str, forward: bool, ptr: int, signal_stride: int, sample_stride: int,
+             length: int, count: int, levels: int = -1, f64: bool = False,
+             stream: int = 0) -> int:
+    """dwt_gpu_1d_batch: full-depth 1D transform of `count` strided signals in
+    place (strides in bytes); returns the levels done."""
+    lv = C.c_int(levels)
+    _check(lib().dwt_gpu_1d_batch(WAVELET_IDS[wavelet], FORWARD if forward else INVERSE,
+                                  F64 if f64 else F32, ptr, signal_stride, sample_stride, length,
+                                  count, C.byref(lv), stream))
+    return lv.value
 
 
 def _storage_dtype(storage: str):

@@ -259,6 +259,7 @@ void init_once() {
   std::call_once(g_init_once, [] {
     g_init_status = configure_line_kernels();
     if (g_init_status == hipSuccess) g_init_status = configure_tail_kernels();
+    if (g_init_status == hipSuccess) g_init_status = configure_lines_kernels();
   });
   if (g_init_status != hipSuccess)
     fail(DWT_GPU_EHIP, "raising the kernels' LDS limits: %s", hipGetErrorString(g_init_status));
@@ -927,6 +928,94 @@ static void run_zero(void* p, int k0, int k1, int stride) {
   HIPCHK(hipStreamSynchronize(st));
 }
 
+// ---- full-depth 1D transforms of many signals ---------------------------------------
+// Signal k, sample i at ptr + k * sig + i * smp (bytes).  Levels as the caller
+// loop over dwt_cdf97_f_ex_stride_s: each level on the current n (L to
+// [0, ceil(n/2)), H to [ceil(n/2), n)) while n >= 2.
+template <typename T>
+static void run_lines(bool fwd, char* ptr, long long sig, long long smp, int N, int count,
+                      int* levels, Wavelet w, hipStream_t st) {
+  init_once();
+  if (N < 0 || count < 0) fail(DWT_GPU_EINVAL, "negative length or count");
+  const int jl = ceil_log2(N);
+  int L = *levels;
+  if (fwd) {
+    if (L < 0 || L > jl) L = jl;
+    *levels = L;
+  } else if (L < 0 || L > jl) {
+    fail(DWT_GPU_EINVAL, "%d levels outside 0..%d", L, jl);
+  }
+  if (L == 0 || count == 0) return;
+  const long long es = sizeof(T);
+  const Coeffs<T> cf = make_coeffs<T>(w, fwd);
+  Scratch& sc = scratch_for(st);
+  const bool aligned = (reinterpret_cast<uintptr_t>(ptr) % es) == 0;
+  std::vector<int> n(L + 1);
+  n[0] = N;
+  for (int j = 1; j <= L; ++j) n[j] = (n[j - 1] + 1) / 2;
+  if (aligned && smp == es && sig % es == 0 && N <= lines_lds_max((int)es)) {
+    // contiguous samples: one workgroup per signal, every level in LDS
+    LAUNCH(launch_lines_lds<T>(fwd, reinterpret_cast<T*>(ptr), sig / es, N, count, L, cf, st));
+    return;
+  }
+  if (aligned && sig == es && smp % es == 0 && smp / es >= count) {
+    // adjacent signals = the columns of a matrix of pitch p
+    T* A = reinterpret_cast<T*>(ptr);
+    const long long p = smp / es;
+    const bool vec = (reinterpret_cast<uintptr_t>(ptr) % (2 * es)) == 0 && p % 2 == 0;
+    const long long cp = (long long)pad_pitch(count);
+    T* D = (T*)sc.get(2, sizeof(T) * (size_t)cp * N);
+    T* P[2] = {nullptr, nullptr};
+    if (L >= 2) P[1] = (T*)sc.get(1, sizeof(T) * (size_t)cp * n[1]);
+    if (L >= 3) P[0] = (T*)sc.get(0, sizeof(T) * (size_t)cp * n[2]);
+    const int h0 = n[1];
+    sc.ensure_side();
+    if (fwd) {
+      // level 0 into scratch (its H rows overlap unread input), copied into
+      // A on the side stream; later levels read scratch and write A directly
+      LAUNCH(launch_col_level<T>(true, A, p, L == 1 ? D : P[1], cp, D, cp, N, count, cf, vec, st));
+      HIPCHK(hipEventRecord(sc.fork, st));
+      HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
+      LAUNCH(launch_copy_rect<T>(D + (L == 1 ? 0 : (long long)h0 * cp), cp, A + (L == 1 ? 0 : (long long)h0 * p),
+                                 p, count, L == 1 ? N : N - h0, sc.side));
+      for (int j = 1; j < L; ++j) {
+        T* out = j == L - 1 ? A : P[(j + 1) % 2];
+        const long long op = j == L - 1 ? p : cp;
+        LAUNCH(launch_col_level<T>(true, P[j % 2], cp, out, op, A, p, n[j], count, cf, true, st));
+      }
+      HIPCHK(hipEventRecord(sc.join, sc.side));
+      HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
+    } else {
+      HIPCHK(hipEventRecord(sc.fork, st));
+      HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
+      LAUNCH(launch_copy_rect<T>(A + (L == 1 ? 0 : (long long)h0 * p), p, D + (L == 1 ? 0 : (long long)h0 * cp),
+                                 cp, count, L == 1 ? N : N - h0, sc.side));
+      HIPCHK(hipEventRecord(sc.join, sc.side));
+      for (int j = L - 1; j >= 1; --j) {
+        const T* llp = j == L - 1 ? A : P[(j + 1) % 2];
+        const long long lp = j == L - 1 ? p : cp;
+        // H rows always come from A: pair-wise loads only if A allows them
+        LAUNCH(launch_col_level<T>(false, llp, lp, A, p, P[j % 2], cp, n[j], count, cf, vec, st));
+      }
+      HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
+      LAUNCH(launch_col_level<T>(false, L == 1 ? D : P[1], cp, D, cp, A, p, N, count, cf, true, st));
+    }
+    return;
+  }
+  // any other layout: one line-parallel launch per level, in place
+  const int lim = line_lds_limit_bytes() / (int)sizeof(T);
+  const int gblocks = 1024;
+  for (int k = 0; k < L; ++k) {
+    const int j = fwd ? k : L - 1 - k;
+    const int m = n[j];
+    T* gt = m > lim ? (T*)sc.get(3, sizeof(T) * (size_t)m * gblocks) : nullptr;
+    if (fwd)
+      LAUNCH(launch_line_pass<T>(true, ptr, ptr, ptr + (long long)n[j + 1] * smp, sig, smp, count, m, cf, gt, gblocks, st));
+    else
+      LAUNCH(launch_line_pass<T>(false, ptr, ptr + (long long)n[j + 1] * smp, ptr, sig, smp, count, m, cf, gt, gblocks, st));
+  }
+}
+
 [[noreturn]] static void abort_with(const char* fn) {
   std::fprintf(stderr, "libdwt (MI355X): %s: %s\n", fn, g_err.c_str());
   std::fflush(stderr);
@@ -1181,6 +1270,31 @@ int dwt_gpu_device_check(void* stream) {
     hipStream_t st = (hipStream_t)stream;
     HIPCHK(hipStreamSynchronize(st));
     if (pyr_failed(scratch_for(st), st)) fail(DWT_GPU_EHIP, "pyramid dependency wait timed out");
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_1d_batch(int wavelet, int direction, int precision, void* ptr, long long signal_stride,
+                     long long sample_stride, int length, int count, int* levels, void* stream) {
+  try {
+    g_err.clear();
+    if (wavelet != DWT_GPU_CDF97 && wavelet != DWT_GPU_CDF53) fail(DWT_GPU_EINVAL, "bad wavelet");
+    if (direction != DWT_GPU_FORWARD && direction != DWT_GPU_INVERSE) fail(DWT_GPU_EINVAL, "bad direction");
+    if (precision != DWT_GPU_F32 && precision != DWT_GPU_F64) fail(DWT_GPU_EINVAL, "bad precision");
+    if (!levels) fail(DWT_GPU_EINVAL, "null levels");
+    if (!ptr) fail(DWT_GPU_EINVAL, "null pointer");
+    if (sample_stride == 0) fail(DWT_GPU_EINVAL, "zero sample stride");
+    void* dptr = device_address(ptr);
+    if (!dptr) fail(DWT_GPU_EMEMORY, "pointer is not device-accessible memory");
+    const Wavelet w = wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53;
+    const bool fwd = direction == DWT_GPU_FORWARD;
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == DWT_GPU_F32)
+      run_lines<float>(fwd, (char*)dptr, signal_stride, sample_stride, length, count, levels, w, st);
+    else
+      run_lines<double>(fwd, (char*)dptr, signal_stride, sample_stride, length, count, levels, w, st);
     return DWT_GPU_OK;
   } catch (const Failure& f) {
     return f.code;

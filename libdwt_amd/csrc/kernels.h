@@ -103,4 +103,21 @@ void launch_pyr(bool fwd, bool vec, int S, const PyrTask* d_tasks, int ntasks, i
                 const PyrSync& sy, const Coeffs<T>& cf, int grid_cap, hipStream_t st);
 void launch_fill_u32(unsigned* p, unsigned v, long long n, hipStream_t st);
 
+// ---- full-depth 1D transforms of many signals (lines.hip) ---------------------------
+// Contiguous samples: one workgroup per signal (signal k at base + k * sig
+// elements), N <= lines_lds_max(sizeof(T)), `levels` levels in LDS.
+int lines_lds_max(int elem);
+template <typename T>
+void launch_lines_lds(bool fwd, T* base, long long sig, int N, int count, int levels,
+                      const Coeffs<T>& cf, hipStream_t st);
+// Signals as the columns of a matrix (count columns): one level on n rows.
+// Forward: a = input rows (pitch ap), b = L rows out (pitch bp), c = H rows
+// out from row ceil(n/2) (pitch cp).  Inverse: a = L rows, b = H rows from
+// row ceil(n/2), c = output rows.  vec: pair-wise access allowed.
+template <typename T>
+void launch_col_level(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c,
+                      long long cp, int n, int count, const Coeffs<T>& cf, bool vec,
+                      hipStream_t st);
+hipError_t configure_lines_kernels();
+
 }  // namespace dwt

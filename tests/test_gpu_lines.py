@@ -1,0 +1,106 @@
+"""GPU: full-depth 1D transforms of many strided signals (dwt_gpu_1d_batch),
+bit-exact to the oracle's caller loop over the single-level line functions,
+for contiguous signals, signals as matrix columns and a generic strided
+layout; and the reference-generated digests of the 256 x 8192 random-walk
+slice (stored as columns)."""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+import libdwt_amd
+from libdwt_amd import workloads
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    assert libdwt_amd.lib().dwt_gpu_init() == 0
+
+
+def oracle_full(o, sig, wavelet, fwd):
+    f = o.f_full_1d if fwd else o.i_full_1d
+    return np.stack([f(r, wavelet) for r in sig])
+
+
+@pytest.mark.parametrize("layout", ["rows", "cols", "strided"])
+@pytest.mark.parametrize("prec", ["s", "d"])
+def test_lines_match_oracle(layout, prec):
+    from oracle.oracle import oracle
+    o = oracle()
+    dt = np.float32 if prec == "s" else np.float64
+    es = np.dtype(dt).itemsize
+    for N in (2, 3, 5, 777, 1000, 8192):
+        count = 37
+        sig = workloads.random_walk(count, N, seed=N).astype(dt)
+        for w in ("cdf97", "cdf53"):
+            say(f"lines {layout} {prec} N={N} {w}")
+            want_f = oracle_full(o, sig, w, True)
+            want_i = oracle_full(o, want_f, w, False)
+            if layout == "rows":  # signal k = row k, padded pitch
+                buf = np.zeros((count, N + 3), dtype=dt)
+                buf[:, :N] = sig
+                t = torch.from_numpy(buf).cuda()
+                args = (t.data_ptr(), (N + 3) * es, es)
+                read = lambda: t.cpu().numpy()[:, :N]
+            elif layout == "cols":  # signal k = column k of an N x (count + 1) matrix
+                buf = np.zeros((N, count + 1), dtype=dt)
+                buf[:, :count] = sig.T
+                t = torch.from_numpy(buf).cuda()
+                args = (t.data_ptr(), es, (count + 1) * es)
+                read = lambda: t.cpu().numpy()[:, :count].T
+            else:  # samples 3 elements apart, each signal in its own region
+                sg = 3 * N + 1
+                buf = np.zeros((count * sg + 3 * N,), dtype=dt)
+                idx = sg * np.arange(count)[:, None] + 3 * np.arange(N)[None, :]
+                buf[idx] = sig
+                t = torch.from_numpy(buf).cuda()
+                args = (t.data_ptr(), sg * es, 3 * es)
+                read = lambda: t.cpu().numpy()[idx]
+            torch.cuda.synchronize()
+            j = libdwt_amd.lines_1d(w, True, args[0], args[1], args[2], N, count, -1, prec == "d")
+            torch.cuda.synchronize()
+            assert j == workloads.ceil_log2(N)
+            np.testing.assert_array_equal(read(), want_f, err_msg="forward")
+            libdwt_amd.lines_1d(w, False, args[0], args[1], args[2], N, count, j, prec == "d")
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(read(), want_i, err_msg="inverse")
+
+
+def test_slice_digests():
+    """The 256 x 8192 random-walk slice as matrix columns through the batched
+    entry point, against the digests the reference produced."""
+    with open(os.path.join(GOLD, "golden_hashes.json")) as fh:
+        rec = json.load(fh)["lines"]["slice_256x8192"]
+    ls = cases.LINE_SLICE
+    sig = workloads.random_walk(ls["count"], ls["length"], ls["seed"])
+    n_sig, N = sig.shape
+    for w in cases.WAVELETS:
+        say(f"slice {w}")
+        mat = torch.from_numpy(np.ascontiguousarray(sig.T)).cuda()
+        torch.cuda.synchronize()
+        j = libdwt_amd.lines_1d(w, True, mat.data_ptr(), 4, n_sig * 4, N, n_sig)
+        torch.cuda.synchronize()
+        assert cases.digest(np.ascontiguousarray(mat.cpu().numpy().T)) == rec[w]["fwd"]
+        libdwt_amd.lines_1d(w, False, mat.data_ptr(), 4, n_sig * 4, N, n_sig, j)
+        torch.cuda.synchronize()
+        assert cases.digest(np.ascontiguousarray(mat.cpu().numpy().T)) == rec[w]["inv"]
+        # the same signals as rows
+        rows = torch.from_numpy(np.ascontiguousarray(sig)).cuda()
+        torch.cuda.synchronize()
+        libdwt_amd.lines_1d(w, True, rows.data_ptr(), N * 4, 4, N, n_sig)
+        torch.cuda.synchronize()
+        assert cases.digest(rows.cpu().numpy()) == rec[w]["fwd"]
