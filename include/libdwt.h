@@ -24,6 +24,9 @@
 #ifndef LIBDWT_H
 #define LIBDWT_H
 
+#include <stdint.h>
+#include <stdio.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -98,6 +101,122 @@ void dwt_cdf97_2i_s(void *ptr, int stride_x, int stride_y, int size_o_big_x, int
 void dwt_cdf53_2i_s(void *ptr, int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
                     int size_i_big_x, int size_i_big_y, int j_max, int decompose_one,
                     int zero_padding);
+
+/* ---- utilities used by programs written against libdwt --------------------
+ * Host-side helpers with the reference's names and behaviour (image
+ * allocation in host memory, test pattern, comparison, PGM output, timers,
+ * logging, strides, subband addressing), so that its example programs build
+ * unchanged against this library.  reference: src/libdwt.h:600-1270 */
+enum dwt_timer_types {
+	DWT_TIME_CLOCK_GETTIME,
+	DWT_TIME_CLOCK_GETTIME_REALTIME,
+	DWT_TIME_CLOCK_GETTIME_MONOTONIC,
+	DWT_TIME_CLOCK_GETTIME_MONOTONIC_RAW,
+	DWT_TIME_CLOCK_GETTIME_PROCESS_CPUTIME_ID,
+	DWT_TIME_CLOCK_GETTIME_THREAD_CPUTIME_ID,
+	DWT_TIME_CLOCK,
+	DWT_TIME_TIMES,
+	DWT_TIME_GETRUSAGE,
+	DWT_TIME_GETRUSAGE_SELF,
+	DWT_TIME_GETRUSAGE_CHILDREN,
+	DWT_TIME_GETRUSAGE_THREAD,
+	DWT_TIME_GETTIMEOFDAY,
+	DWT_TIME_IOCTL_RTC,
+	DWT_TIME_AUTOSELECT
+};
+typedef int64_t dwt_clock_t;
+enum dwt_util_loglevel { LOG_NONE = 0, LOG_DBG, LOG_INFO, LOG_WARN, LOG_ERR, LOG_TEST };
+enum dwt_subbands { DWT_LL, DWT_HL, DWT_LH, DWT_HH };
+
+void dwt_util_test_image_fill_s(void *ptr, int stride_x, int stride_y, int size_i_big_x,
+                                int size_i_big_y, int rand);
+void dwt_util_test_image_fill_d(void *ptr, int stride_x, int stride_y, int size_i_big_x,
+                                int size_i_big_y, int rand);
+void dwt_util_test_image_zero_s(void *ptr, int stride_x, int stride_y, int size_i_big_x,
+                                int size_i_big_y);
+void dwt_util_alloc_image(void **pptr, int stride_x, int stride_y, int size_o_big_x,
+                          int size_o_big_y);
+void dwt_util_free_image(void **pptr);
+int dwt_util_compare_s(void *ptr1, void *ptr2, int stride_x, int stride_y, int size_i_big_x,
+                       int size_i_big_y);
+int dwt_util_compare_d(void *ptr1, void *ptr2, int stride_x, int stride_y, int size_i_big_x,
+                       int size_i_big_y);
+const char *dwt_util_version(void);
+const char *dwt_util_arch(void);
+int dwt_util_clock_autoselect(void);
+dwt_clock_t dwt_util_get_frequency(int type);
+dwt_clock_t dwt_util_get_clock(int type);
+int dwt_util_get_num_threads(void);
+int dwt_util_get_num_workers(void);
+int dwt_util_get_max_threads(void);
+int dwt_util_get_thread_num(void);
+void dwt_util_init(void);
+void dwt_util_finish(void);
+void dwt_util_set_accel(int accel_type);
+int dwt_util_save_to_pgm_s(const char *filename, float max_value, const void *ptr, int stride_x,
+                           int stride_y, int size_i_big_x, int size_i_big_y);
+int dwt_util_save_to_pgm_d(const char *filename, double max_value, const void *ptr, int stride_x,
+                           int stride_y, int size_i_big_x, int size_i_big_y);
+int dwt_util_log(enum dwt_util_loglevel level, const char *format, ...);
+int dwt_util_is_aligned_8(const void *ptr);
+const char *dwt_util_node(void);
+const char *dwt_util_appname(void);
+int dwt_util_is_prime(int N);
+int dwt_util_next_prime(int N);
+int dwt_util_get_opt_stride(int min_stride);
+int dwt_util_get_stride(int min_stride, int opt);
+void dwt_util_subband(void *ptr, int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
+                      int size_i_big_x, int size_i_big_y, int j_max, enum dwt_subbands band,
+                      void **dst_ptr, int *dst_size_x, int *dst_size_y);
+void dwt_util_subband_s(void *ptr, int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
+                        int size_i_big_x, int size_i_big_y, int j_max, enum dwt_subbands band,
+                        void **dst_ptr, int *dst_size_x, int *dst_size_y);
+void dwt_util_subband_d(void *ptr, int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
+                        int size_i_big_x, int size_i_big_y, int j_max, enum dwt_subbands band,
+                        void **dst_ptr, int *dst_size_x, int *dst_size_y);
+float *dwt_util_addr_coeff_s(void *ptr, int y, int x, int stride_x, int stride_y);
+double *dwt_util_addr_coeff_d(void *ptr, int y, int x, int stride_x, int stride_y);
+void dwt_util_conv_show_s(const void *src, void *dst, int stride_x, int stride_y,
+                          int size_i_big_x, int size_i_big_y);
+void dwt_util_conv_show_d(const void *src, void *dst, int stride_x, int stride_y,
+                          int size_i_big_x, int size_i_big_y);
+void dwt_util_set_num_threads(int num_threads);
+void dwt_util_set_num_workers(int num_workers);
+void dwt_util_abort(void);
+/* Performance harness (reference: src/libdwt.h:1250-1412); the M images of a
+ * performance test live in device memory here. */
+enum dwt_array { DWT_ARR_SIMPLE, DWT_ARR_SPARSE, DWT_ARR_PACKED };
+void dwt_util_perf_cdf97_2_s(int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
+                             int size_i_big_x, int size_i_big_y, int j_max, int decompose_one,
+                             int zero_padding, int M, int N, int clock_type, float *fwd_secs,
+                             float *inv_secs);
+void dwt_util_perf_cdf97_2_d(int stride_x, int stride_y, int size_o_big_x, int size_o_big_y,
+                             int size_i_big_x, int size_i_big_y, int j_max, int decompose_one,
+                             int zero_padding, int M, int N, int clock_type, double *fwd_secs,
+                             double *inv_secs);
+int dwt_util_pow2_ceil_log2(int x);
+void dwt_util_get_sizes_s(enum dwt_array array_type, int size_x, int size_y, int opt_stride,
+                          int *stride_x, int *stride_y, int *size_o_big_x, int *size_o_big_y,
+                          int *size_i_big_x, int *size_i_big_y);
+void dwt_util_get_sizes_d(enum dwt_array array_type, int size_x, int size_y, int opt_stride,
+                          int *stride_x, int *stride_y, int *size_o_big_x, int *size_o_big_y,
+                          int *size_i_big_x, int *size_i_big_y);
+void dwt_util_measure_perf_cdf97_1_s(enum dwt_array array_type, int min_x, int max_x,
+                                     int opt_stride, int j_max, int decompose_one,
+                                     int zero_padding, int M, int N, int clock_type,
+                                     FILE *fwd_plot_data, FILE *inv_plot_data);
+void dwt_util_measure_perf_cdf97_1_d(enum dwt_array array_type, int min_x, int max_x,
+                                     int opt_stride, int j_max, int decompose_one,
+                                     int zero_padding, int M, int N, int clock_type,
+                                     FILE *fwd_plot_data, FILE *inv_plot_data);
+void dwt_util_measure_perf_cdf97_2_s(enum dwt_array array_type, int min_x, int max_x,
+                                     int opt_stride, int j_max, int decompose_one,
+                                     int zero_padding, int M, int N, int clock_type,
+                                     FILE *fwd_plot_data, FILE *inv_plot_data);
+void dwt_util_measure_perf_cdf97_2_d(enum dwt_array array_type, int min_x, int max_x,
+                                     int opt_stride, int j_max, int decompose_one,
+                                     int zero_padding, int M, int N, int clock_type,
+                                     FILE *fwd_plot_data, FILE *inv_plot_data);
 
 /* ===========================================================================
  * Extensions (no counterpart in the reference).  They return 0 on success or

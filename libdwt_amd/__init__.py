@@ -43,8 +43,8 @@ def header_functions() -> list[str]:
     with open(HEADER) as fh:
         text = fh.read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return re.findall(r"^\s*(?:const\s+)?(?:void|int|long long|char)\s*\*?\s*(\w+)\s*\(", text,
-                      flags=re.M)
+    return re.findall(r"^\s*(?:const\s+)?(?:void|int|long long|char|float|double|dwt_clock_t)"
+                      r"\s*\*?\s*(\w+)\s*\(", text, flags=re.M)
 
 
 def lib() -> C.CDLL:
