@@ -300,6 +300,7 @@ static void run_pyr(bool fwd, std::vector<PyrTask>& tasks, int S, bool vec, cons
                     Scratch& sc, hipStream_t st) {
   for (auto& t : tasks) t.S = S;
   const int ntasks = (int)tasks.size();
+  if (ntasks > kPyrMaxTasks) fail(DWT_GPU_EINVAL, "a pyramid holds at most %d levels", kPyrMaxTasks);
   const int nitems = pyr_layout(tasks.data(), ntasks, cf.nsteps);
   if (nitems == 0) return;
   const int ncnt = tasks.back().cnt0 + tasks.back().nseg;

@@ -75,7 +75,8 @@ struct PyrTask {
   int item0, cnt0;
   int dep, reserved;
 };
-// Device synchronisation state of a launch: ctrl[0] next item, ctrl[1]
+constexpr int kPyrMaxTasks = 32;  // tasks of one pyramid launch
+// Device synchronisation state of a launch: ctrl[0] unused, ctrl[1]
 // finished workgroups; cnt[ncnt] finished workgroups per (task, segment), all
 // zero before a launch (the launch leaves them zero); deps[ncnt] producer
 // segment range of each (task, segment); err set by a wait that ran longer

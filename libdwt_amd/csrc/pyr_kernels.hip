@@ -4,9 +4,9 @@
 // A launch gets a table of tasks, one per (image, level), in an order in
 // which every task comes after the task that produces its input.  A task is
 // cut into work items: one workgroup (four wavefronts, four adjacent strips)
-// over one segment of S rows, items numbered segment-major.  Workgroups take
-// items in that order from an atomic ticket counter, so that items are
-// started in table order whatever the dispatch order of the workgroups.
+// over one segment of S rows, items numbered segment-major.  Workgroup b takes
+// items b, b + grid, b + 2 grid, ... in increasing order; the grid never
+// exceeds the number of workgroups the GPU holds at once.
 //
 // Dataflow instead of a grid barrier between levels: when a workgroup has
 // finished its item it increments the counter of its (task, segment); an item
@@ -14,8 +14,18 @@
 // reads (the range is computed on the host from the same row arithmetic as
 // the tile body, halo and mirrored rows included) has all its workgroups
 // counted.  Levels therefore overlap: the first segments of level j + 1 run
-// while level j finishes its last ones, and the small levels cost a few
-// dependent memory round trips instead of a launch each.
+// while level j finishes its last ones.
+//
+// Progress: an item only waits on items with smaller numbers.  The smallest
+// unfinished item belongs to a workgroup that is processing it (its own
+// earlier items are smaller, hence finished) and whose dependencies are all
+// smaller, hence finished; all workgroups of the grid fit on the GPU together,
+// so that workgroup is running (other kernels on the GPU end on their own).
+// Every wait is still bounded by the GPU's constant-rate clock: past the limit
+// the workgroup records an error flag and goes on (its results are then wrong
+// and the host reports the flag), so no launch can hang.  A single atomic
+// ticket counter was measured to serialise the start of a level's items (one
+// ticket per ~11 ns across XCDs), hence the static assignment.
 //
 // Coherence across the eight XCDs without cache maintenance.  A release /
 // acquire pair at agent scope would write back and invalidate a whole XCD's
@@ -31,43 +41,27 @@
 //     plain loads miss down to memory and see the final values.
 // The detail subbands are not read inside a launch and use plain stores.
 //
-// Progress: an item only waits on items with smaller tickets, which were
-// taken by workgroups that are running, so the lowest unfinished ticket can
-// always proceed and the launch cannot deadlock, whatever the grid size.  A
-// wait is still bounded by the GPU's constant-rate clock: past the limit the
-// workgroup records an error flag and goes on (its results are then wrong and
-// the host reports the flag), so no launch can hang.
-//
-// The last workgroup to finish resets the ticket and segment counters to
-// zero, so the next launch on the same buffers (or a graph replay) starts
-// clean without a memset.
+// The task table is copied into LDS by every workgroup once, so finding an
+// item's task costs a few LDS reads.  The last workgroup to finish resets the
+// segment counters to zero, so the next launch on the same buffers (or a
+// graph replay) starts clean without a memset.
 #include "tile_body.h"
 #include "kernels.h"
 
 #include <algorithm>
+#include <cstddef>
 #include <climits>
 #include <mutex>
 
 namespace dwt {
 
-__device__ __forceinline__ int pyr_find_task(const PyrTask* __restrict__ tasks, int ntasks, int item) {
-  int lo = 0, hi = ntasks - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (tasks[mid].item0 <= item) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
 // Control flow of the persistent loop is kept wave-uniform.  A branch on
-// threadIdx.x == 0 around the ticket and counter atomics lets the compiler
-// merge the end of one iteration with the start of the next into one
-// lane-divergent region, and then lane 0 leaves an inner loop while the
-// wavefront's other lanes go on through the barrier with the previous
-// ticket, for ever.  So every atomic below is executed by all 64 lanes of
-// wave 0 (only lane 0 adds a non-zero amount) and every value a branch or
-// loop depends on is made uniform with readfirstlane.
+// threadIdx.x == 0 around atomics at the end and start of an iteration lets
+// the compiler merge them into one lane-divergent region, and then lane 0
+// leaves an inner loop while the wavefront's other lanes go on through the
+// barrier with stale state, for ever.  So every atomic below is executed by
+// all 64 lanes of wave 0 (only lane 0 adds a non-zero amount) and every value
+// a branch or loop depends on is made uniform with readfirstlane.
 __device__ __forceinline__ int wave_index() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 
 __device__ __forceinline__ unsigned wave_fetch_add(unsigned* p, unsigned v) {
@@ -95,14 +89,6 @@ __device__ __forceinline__ void pyr_wait(const PyrSync& sy, int base, int2 r, un
     }
   }
   if (timed_out) __hip_atomic_store(sy.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Take the next item (every thread gets it); -1 when the table is exhausted.
-__device__ __forceinline__ int pyr_next(const PyrSync& sy, int nitems, int* s_item) {
-  if (wave_index() == 0) *s_item = (int)wave_fetch_add(sy.ctrl, 1u);  // same value from every lane
-  __syncthreads();
-  const int item = __builtin_amdgcn_readfirstlane(*s_item);
-  return item < nitems ? item : -1;
 }
 
 // Optional per-item timeline (wave 0 of the workgroup writes it).
@@ -136,78 +122,86 @@ __device__ __forceinline__ void pyr_finish(const PyrSync& sy, int* s_flag) {
   if (threadIdx.x < 2) __hip_atomic_store(sy.ctrl + threadIdx.x, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Forward: task = one level; src = level input (W x H), out = LL of the next
-// level, det = detail subbands at their Mallat positions.
-template <typename T, int NS, bool VEC, int SS>
-__global__ void __launch_bounds__(256)
-pyr_fwd_kernel(const PyrTask* __restrict__ tasks, int ntasks, int nitems, PyrSync sy, Coeffs<T> cf) {
-  __shared__ int s_item, s_flag;
+constexpr int kTaskWords = sizeof(PyrTask) / 4;
+static_assert(sizeof(PyrTask) % 4 == 0, "task words");
+
+// A task's fields from the LDS copy, as wave-uniform values.
+__device__ __forceinline__ PyrTask uniform_task(const Lds<int>* w) {
+  PyrTask r;
+  int* o = reinterpret_cast<int*>(&r);
+#pragma unroll
+  for (int i = 0; i < kTaskWords; ++i) o[i] = __builtin_amdgcn_readfirstlane(w[i]);
+  return r;
+}
+
+// The persistent loop of both directions.  Forward: task = one level; src =
+// level input (W x H), out = LL of the next level, det = detail subbands at
+// their Mallat positions.  Inverse: src = LL of the coarser level, det =
+// details at their Mallat positions of the W x H output region, out = that
+// region.
+template <bool FWD, typename T, int NS, bool VEC, int SS>
+__device__ __forceinline__ void pyr_body(const PyrTask* __restrict__ tasks, int ntasks, int nitems,
+                                         const PyrSync& sy, const Coeffs<T>& cf) {
+  __shared__ int s_tasks[kPyrMaxTasks * kTaskWords];
+  __shared__ int s_flag;
+  Lds<int>* const lt = (Lds<int>*)s_tasks;  // generic -> LDS address-space cast
+  {
+    const Gbl<int>* g = gload_ptr(reinterpret_cast<const int*>(tasks));
+    for (int i = threadIdx.x; i < ntasks * kTaskWords; i += blockDim.x) lt[i] = g[i];
+  }
+  __syncthreads();
   const int wave = wave_index();
-  // a workgroup cannot take more than nitems items: the bound only makes
-  // termination independent of the ticket counter
-  for (int iter = 0; iter < nitems; ++iter) {
-    const int item = pyr_next(sy, nitems, &s_item);
-    if (item < 0) break;
+  for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
     pyr_trace(sy, item, 0);
     pyr_trace(sy, item, 3);
-    const PyrTask& tk = tasks[pyr_find_task(tasks, ntasks, item)];
+    int lo = 0, hi = ntasks - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (__builtin_amdgcn_readfirstlane(lt[mid * kTaskWords + (int)(offsetof(PyrTask, item0) / 4)]) <= item)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const PyrTask tk = uniform_task(lt + lo * kTaskWords);
     const int local = item - tk.item0;
     const int s = local / tk.ngroups, g = local - s * tk.ngroups;
     if (tk.dep >= 0) {
       if (wave == 0) {
-        const PyrTask& pt = tasks[tk.dep];
+        const PyrTask pt = uniform_task(lt + tk.dep * kTaskWords);
         pyr_wait(sy, pt.cnt0, sy.deps[tk.cnt0 + s], (unsigned)pt.ngroups);
       }
       __syncthreads();
     }
     pyr_trace(sy, item, 1);
     const int strip = g * kWavesPerBlock + wave;
-    if (strip < tk.nstrips)
-      fwd_tile<T, NS, -SS>(strip, s, SS, tk.W, tk.H,
-                           GlobalRows<T, VEC>{static_cast<const T*>(tk.src), tk.sp, tk.H},
-                           CoherentStore<T>{static_cast<T*>(tk.out), tk.op},
-                           GlobalDet<T>{static_cast<T*>(tk.det), tk.dp}, cf);
+    if (strip < tk.nstrips) {
+      if constexpr (FWD)
+        fwd_tile<T, NS, -SS>(strip, s, SS, tk.W, tk.H,
+                             GlobalRows<T, VEC>{static_cast<const T*>(tk.src), tk.sp, tk.H},
+                             CoherentStore<T>{static_cast<T*>(tk.out), tk.op},
+                             GlobalDet<T>{static_cast<T*>(tk.det), tk.dp}, cf);
+      else
+        inv_tile<T, NS, -SS>(strip, s, SS, tk.W, tk.H,
+                             GlobalLoad<T>{static_cast<const T*>(tk.src), tk.sp},
+                             GlobalDet<T>{static_cast<T*>(tk.det), tk.dp},
+                             CoherentOut<T, VEC>{static_cast<T*>(tk.out), tk.op}, cf);
+    }
     pyr_done(sy, tk.cnt0 + s);
     pyr_trace(sy, item, 2);
   }
   pyr_finish(sy, &s_flag);
 }
 
-// Inverse: task = one level; src = LL of the coarser level, det = details at
-// their Mallat positions of the W x H output region, out = that region.
+template <typename T, int NS, bool VEC, int SS>
+__global__ void __launch_bounds__(256)
+pyr_fwd_kernel(const PyrTask* __restrict__ tasks, int ntasks, int nitems, PyrSync sy, Coeffs<T> cf) {
+  pyr_body<true, T, NS, VEC, SS>(tasks, ntasks, nitems, sy, cf);
+}
+
 template <typename T, int NS, bool VEC, int SS>
 __global__ void __launch_bounds__(256)
 pyr_inv_kernel(const PyrTask* __restrict__ tasks, int ntasks, int nitems, PyrSync sy, Coeffs<T> cf) {
-  __shared__ int s_item, s_flag;
-  const int wave = wave_index();
-  // a workgroup cannot take more than nitems items: the bound only makes
-  // termination independent of the ticket counter
-  for (int iter = 0; iter < nitems; ++iter) {
-    const int item = pyr_next(sy, nitems, &s_item);
-    if (item < 0) break;
-    pyr_trace(sy, item, 0);
-    pyr_trace(sy, item, 3);
-    const PyrTask& tk = tasks[pyr_find_task(tasks, ntasks, item)];
-    const int local = item - tk.item0;
-    const int s = local / tk.ngroups, g = local - s * tk.ngroups;
-    if (tk.dep >= 0) {
-      if (wave == 0) {
-        const PyrTask& pt = tasks[tk.dep];
-        pyr_wait(sy, pt.cnt0, sy.deps[tk.cnt0 + s], (unsigned)pt.ngroups);
-      }
-      __syncthreads();
-    }
-    pyr_trace(sy, item, 1);
-    const int strip = g * kWavesPerBlock + wave;
-    if (strip < tk.nstrips)
-      inv_tile<T, NS, -SS>(strip, s, SS, tk.W, tk.H,
-                           GlobalLoad<T>{static_cast<const T*>(tk.src), tk.sp},
-                           GlobalDet<T>{static_cast<T*>(tk.det), tk.dp},
-                           CoherentOut<T, VEC>{static_cast<T*>(tk.out), tk.op}, cf);
-    pyr_done(sy, tk.cnt0 + s);
-    pyr_trace(sy, item, 2);
-  }
-  pyr_finish(sy, &s_flag);
+  pyr_body<false, T, NS, VEC, SS>(tasks, ntasks, nitems, sy, cf);
 }
 
 __global__ void fill_u32_kernel(unsigned* p, unsigned v, long long n) {
