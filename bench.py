@@ -532,6 +532,24 @@ def main():
                 kernels[f"{name}{lev}"] = {"launches": n, "avg_us": round(1000 * ms / n, 3)}
     libdwt_amd.profile_reset()
 
+    # plain-copy ceiling on the same stream: the same bytes as level 0 (read
+    # W*H floats, write W*H floats), back to back, HIP events around 20 copies
+    ca = torch.empty(W * H, device=dev).fill_(1.0)
+    cb_ = torch.empty_like(ca)
+    with torch.cuda.stream(stream):
+        for _ in range(3):
+            cb_.copy_(ca)
+        c0 = torch.cuda.Event(enable_timing=True)
+        c1 = torch.cuda.Event(enable_timing=True)
+        c0.record(stream)
+        for _ in range(20):
+            cb_.copy_(ca)
+        c1.record(stream)
+    torch.cuda.synchronize()
+    copy_us = c0.elapsed_time(c1) * 1000 / 20
+    copy_gbs = 8 * W * H / (copy_us * 1e-6) / 1e9
+    del ca, cb_
+
     # dominant kernel: level-0 forward lifting kernel, 4 B read + 4 B write per pixel
     lv0 = kernels.get("fwd_level0")
     alg_bytes = 8 * W * H
@@ -565,6 +583,9 @@ def main():
             "traffic": measured_traffic(W, H, wv),
             "algorithmic_bytes": alg_bytes,
             "avg_launch_us": lv0["avg_us"] if lv0 else None,
+            "copy_ceiling": {"gbs": round(copy_gbs, 1), "avg_us": round(copy_us, 3),
+                             "bytes": 8 * W * H,
+                             "frac_of_copy": round(achieved / copy_gbs, 4) if achieved else None},
         },
         "cold": ({"ms_per_step_median": round(cold[len(cold) // 2], 5),
                   "value": round(W * H / (cold[len(cold) // 2] * 1e-3) / 1e6, 3),

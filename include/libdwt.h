@@ -273,9 +273,10 @@ int dwt_gpu_tile_get(int forward, int width, int height, int wavelet, int *seg_r
                      int *prefetch_pairs);
 
 /* The small levels of a transform run as one single-workgroup launch that
- * keeps the whole small pyramid in LDS, from the first level whose region is
- * at most max_region_bytes (default 16 KiB, at most 75 KiB; 0 disables it).
- * An image that small is transformed completely by that one launch. */
+ * keeps the whole small pyramid in LDS, from the first level j >= 1 whose
+ * region is at most max_region_bytes (default 32 KiB, at most 48 KiB; 0
+ * disables it).  An image of at most twice that is transformed completely by
+ * that one launch. */
 int dwt_gpu_tail_set(long long max_region_bytes);
 
 /* Schedule of the levels after level 0 on the packed path: 0 (default) runs

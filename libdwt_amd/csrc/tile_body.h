@@ -531,6 +531,13 @@ struct CoherentOut {  // two adjacent output samples, written through
   }
 };
 template <typename T>
+struct LdsDet {  // detail subbands in LDS
+  Lds<T>* p;
+  int pitch;
+  __device__ __forceinline__ void store(int, int r, int c, T v) const { p[r * pitch + c] = v; }
+  __device__ __forceinline__ T load(int, int r, int c) const { return p[r * pitch + c]; }
+};
+template <typename T>
 struct LdsOut {
   Lds<T>* p;
   int pitch;

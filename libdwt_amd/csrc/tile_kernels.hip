@@ -38,161 +38,106 @@ inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ 
 // ---- block kernels (one workgroup, a whole small pyramid in LDS) --------------
 // The small levels of a transform are latency-bound: every kernel boundary on
 // this multi-XCD GPU writes back and invalidates L2, and a level of a few KB
-// cannot fill the machine.  A block kernel instead moves the whole small Mallat
-// region (the LL input of the first small level, or all coefficients of the
-// small levels for the inverse) into LDS with one cooperative sweep, runs every
-// remaining level there, and writes the region back with one sweep: two global
-// round trips for all small levels.  Because the whole region is read before
-// anything is written, it may also work in place (an image small enough to fit
-// is transformed completely by one launch).
-//
-// Within LDS a level is computed line-parallel by the whole workgroup, each
-// lifting step a barrier-separated sweep (x + c * (l + r), a missing neighbour
-// at a line end replaced by the existing one); the parity scaling is folded
-// into the move between Mallat and interleaved order through the second LDS
-// buffer.  Rows first, then columns, in both directions.
+// cannot fill the machine.  A block kernel does all levels from the first
+// small one on in one workgroup, with the same tile bodies as the level
+// kernels but every intermediate in LDS:
+//   * C (W0 x H0): the region's coefficients in Mallat layout;
+//   * P (ceil(W0/2) x ceil(H0/2)) and Q (ceil(W0/4) x ceil(H0/4)): the LL of
+//     odd and even levels.
+// Forward: level 0 reads the region from global memory, later levels from P /
+// Q; details go to C, the last LL to C; C is written back once.  Inverse: the
+// region is loaded into C once; each level reads its LL from C (coarsest) or
+// P / Q and its details from C, writing P / Q, and the finest level writes
+// the output to global memory.  The region is read completely before any of
+// it is written, so src may equal dst (an image this small is transformed by
+// one launch).  Sixteen wavefronts take the level's tiles in turn; a barrier
+// separates levels.
 constexpr int kBlockThreads = 1024;
+constexpr int kBlockRows = 8;  // segment rows of the tiles (single shot)
 
-// Lifting steps on the H rows (length W >= 2) of buf: wave w takes rows
-// w, w + nwaves, ...; lane l takes the samples of the step's parity at
-// positions par + 2l, par + 2l + 128, ...
 template <typename T>
-__device__ __forceinline__ void lds_lift_rows(Lds<T>* buf, int pitch, int W, int H,
-                                              const Coeffs<T>& cf) {
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int st = 0; st < cf.nsteps; ++st) {
-    const int par = (cf.first_parity + st) & 1;
-    const T c = cf.c[st];
-    const T c2 = T(2) * c;
-    for (int y = wave; y < H; y += nw) {
-      Lds<T>* row = buf + y * pitch;
-      for (int i = par + 2 * lane; i < W; i += 2 * kWave) {
-        T t;
-        if (i > 0 && i + 1 < W) {
-          t = row[i - 1] + row[i + 1];
-          t = c * t;
-        } else {
-          t = c2 * (i > 0 ? row[i - 1] : row[i + 1]);
-        }
-        row[i] = row[i] + t;
-      }
-    }
-    __syncthreads();
+struct BlockBufs {
+  Lds<T>* C;
+  Lds<T>* P;
+  Lds<T>* Q;
+  int W0, H0, pp, pq;
+  __device__ BlockBufs(Lds<T>* base, int W0_, int H0_) : W0(W0_), H0(H0_) {
+    pp = (W0 + 1) >> 1;
+    pq = (W0 + 3) >> 2;
+    C = base;
+    P = C + W0 * H0;
+    Q = P + pp * ((H0 + 1) >> 1);
   }
-}
+  // buffer and pitch of LL_j, 1 <= j
+  __device__ Lds<T>* ll(int j) const { return (j & 1) ? P : Q; }
+  __device__ int llp(int j) const { return (j & 1) ? pp : pq; }
+};
 
-// Lifting steps on the W columns (length H >= 2) of buf: lane l takes columns
-// l, l + 64, ... (consecutive lanes, consecutive LDS banks); wave w takes the
-// step's positions par + 2w, par + 2w + 2 nwaves, ...
-template <typename T>
-__device__ __forceinline__ void lds_lift_cols(Lds<T>* buf, int pitch, int W, int H,
-                                              const Coeffs<T>& cf) {
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int st = 0; st < cf.nsteps; ++st) {
-    const int par = (cf.first_parity + st) & 1;
-    const T c = cf.c[st];
-    const T c2 = T(2) * c;
-    for (int x = lane; x < W; x += kWave) {
-      for (int i = par + 2 * wave; i < H; i += 2 * nw) {
-        Lds<T>* e = buf + i * pitch + x;
-        T t;
-        if (i > 0 && i + 1 < H) {
-          t = e[-pitch] + e[pitch];
-          t = c * t;
-        } else {
-          t = c2 * (i > 0 ? e[-pitch] : e[pitch]);
-        }
-        e[0] = e[0] + t;
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// Permutation with the parity scaling folded in, over the W x H region.
-// Forward (deinterleave): out[.., j] = in[.., i] * (i odd ? so : se) with even
-// i to j = i/2 and odd i to j = ceil(N/2) + i/2.  Inverse (interleave):
-// out[.., j] = in[.., i] * (j odd ? so : se) with i < ceil(N/2) to j = 2i, the
-// rest to j = 2(i - ceil(N/2)) + 1.  Along rows (ROWS) or columns.
-template <typename T, bool ROWS, bool DEINTERLEAVE>
-__device__ __forceinline__ void lds_permute_scale(const Lds<T>* in, Lds<T>* out, int pitch, int W,
-                                                  int H, T se, T so) {
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const int N = ROWS ? W : H, half = (N + 1) >> 1;
-  for (int y = wave; y < H; y += nw) {
-    for (int x = lane; x < W; x += kWave) {
-      const int i = ROWS ? x : y;
-      int j;
-      bool odd;
-      if (DEINTERLEAVE) {
-        j = (i & 1) ? half + (i >> 1) : (i >> 1);
-        odd = i & 1;
-      } else {
-        j = i < half ? 2 * i : 2 * (i - half) + 1;
-        odd = j & 1;
-      }
-      const int yo = ROWS ? y : j, xo = ROWS ? j : x;
-      out[yo * pitch + xo] = in[y * pitch + x] * (odd ? so : se);
-    }
-  }
-  __syncthreads();
-}
-
-template <typename T>
-__device__ __forceinline__ void block_load(const T* __restrict__ src, long long sp, Lds<T>* b,
-                                           int W, int H) {
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int y = wave; y < H; y += nw)
-    for (int x = lane; x < W; x += kWave) b[y * W + x] = src[(long long)y * sp + x];
-  __syncthreads();
-}
-
-template <typename T>
-__device__ __forceinline__ void block_store(const Lds<T>* b, T* __restrict__ dst, long long dp,
-                                            int W, int H) {
-  const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int y = wave; y < H; y += nw)
-    for (int x = lane; x < W; x += kWave) st_global(dst + (long long)y * dp + x, (T)b[y * W + x]);
-}
-
-// Forward: levels 0..nlev-1 of the W0 x H0 region read from src (pitch sp),
-// result (Mallat layout) to dst (pitch dp); src may equal dst.
-template <typename T>
+template <typename T, int NS>
 __global__ void __launch_bounds__(kBlockThreads)
 block_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, long long dp, int W0,
                  int H0, int nlev, Coeffs<T> cf) {
   extern __shared__ __align__(16) unsigned char blk_smem[];
-  Lds<T>* const b0 = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
-  Lds<T>* const b1 = b0 + W0 * H0;
-  block_load(src, sp, b0, W0, H0);
+  const BlockBufs<T> b((Lds<T>*)blk_smem, W0, H0);  // generic -> LDS address-space cast
+  constexpr int CORE = 2 * kWave - 2 * NS;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = kBlockThreads / kWave;
+  const LdsDet<T> det{b.C, W0};
   for (int lev = 0; lev < nlev; ++lev) {
     const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
-    lds_lift_rows(b0, W0, W, H, cf);  // rows: lift, then scale into Mallat order
-    lds_permute_scale<T, true, true>(b0, b1, W0, W, H, cf.se, cf.so);
-    lds_lift_cols(b1, W0, W, H, cf);  // columns
-    lds_permute_scale<T, false, true>(b1, b0, W0, W, H, cf.se, cf.so);
+    const int nstrips = (W + CORE - 1) / CORE, ntiles = nstrips * ((H + kBlockRows - 1) / kBlockRows);
+    const bool last = lev == nlev - 1;
+    const LdsStore<T> ll{last ? b.C : b.ll(lev + 1), last ? W0 : b.llp(lev + 1)};
+    for (int t = wave; t < ntiles; t += nw) {
+      const int seg = t / nstrips, strip = t - seg * nstrips;
+      if (lev == 0)
+        fwd_tile<T, NS, -kBlockRows>(strip, seg, kBlockRows, W, H, GlobalRows<T, false>{src, sp, H}, ll,
+                                     det, cf);
+      else
+        fwd_tile<T, NS, -kBlockRows>(strip, seg, kBlockRows, W, H,
+                                     LdsRows<T>{b.ll(lev), b.llp(lev), H}, ll, det, cf);
+    }
+    __syncthreads();
   }
-  block_store(b0, dst, dp, W0, H0);
+  Gbl<T>* g = gstore_ptr(dst);
+  for (int i = threadIdx.x; i < W0 * H0; i += kBlockThreads) {
+    const int y = i / W0, x = i - y * W0;
+    g[(long long)y * dp + x] = b.C[i];
+  }
 }
 
-// Inverse: the nlev levels whose finest output is W0 x H0; coefficients of the
-// W0 x H0 region from src (pitch sp), output to dst (pitch dp); src may equal dst.
-template <typename T>
+template <typename T, int NS>
 __global__ void __launch_bounds__(kBlockThreads)
 block_inv_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, long long dp, int W0,
                  int H0, int nlev, Coeffs<T> cf) {
   extern __shared__ __align__(16) unsigned char blk_smem[];
-  Lds<T>* const b0 = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
-  Lds<T>* const b1 = b0 + W0 * H0;
-  block_load(src, sp, b0, W0, H0);
+  const BlockBufs<T> b((Lds<T>*)blk_smem, W0, H0);  // generic -> LDS address-space cast
+  constexpr int CORE = 2 * kWave - 2 * NS;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = kBlockThreads / kWave;
+  {
+    const Gbl<T>* g = gload_ptr(src);
+    for (int i = threadIdx.x; i < W0 * H0; i += kBlockThreads) {
+      const int y = i / W0, x = i - y * W0;
+      b.C[i] = g[(long long)y * sp + x];
+    }
+  }
+  __syncthreads();
+  const LdsDet<T> det{b.C, W0};
   for (int lev = nlev - 1; lev >= 0; --lev) {
     const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
-    lds_permute_scale<T, true, false>(b0, b1, W0, W, H, cf.se, cf.so);  // rows: order, scale,
-    lds_lift_rows(b1, W0, W, H, cf);                                    // lift
-    lds_permute_scale<T, false, false>(b1, b0, W0, W, H, cf.se, cf.so);  // columns
-    lds_lift_cols(b0, W0, W, H, cf);
+    const int nstrips = (W + CORE - 1) / CORE, ntiles = nstrips * ((H + kBlockRows - 1) / kBlockRows);
+    const bool coarsest = lev == nlev - 1;
+    const LdsLoad<T> ll{coarsest ? b.C : b.ll(lev + 1), coarsest ? W0 : b.llp(lev + 1)};
+    for (int t = wave; t < ntiles; t += nw) {
+      const int seg = t / nstrips, strip = t - seg * nstrips;
+      if (lev == 0)
+        inv_tile<T, NS, -kBlockRows>(strip, seg, kBlockRows, W, H, ll, det,
+                                     GlobalOut<T, false>{dst, dp}, cf);
+      else
+        inv_tile<T, NS, -kBlockRows>(strip, seg, kBlockRows, W, H, ll, det,
+                                     LdsOut<T>{b.ll(lev), b.llp(lev)}, cf);
+    }
+    __syncthreads();
   }
-  block_store(b0, dst, dp, W0, H0);
 }
 
 template <typename T>
@@ -333,30 +278,45 @@ void launch_inv_level(const T* ll, long long llp, const T* det, long long dp, T*
 }
 
 // ---- block (tail) launchers --------------------------------------------------------
-// The block kernel takes over from the first level whose region fits the LDS
-// budget (two buffers of W x H elements).  Level 0 may be included: the block
-// kernel reads its whole region before writing, so it works in place.
-static long long g_block_bytes = 16 * 1024;  // one LDS buffer
-constexpr int kBlockLdsMax = 150 * 1024;
+// The block kernel takes over from the first level j >= 1 whose W x H region
+// is at most g_block_bytes, or from level 0 when the whole image is at most
+// twice that: one workgroup then replaces every launch of the transform
+// (measured on MI355X: 128 x 128 CDF 9/7 J=7 fwd+inv 65 us in one block vs
+// 95 us with a level-0 launch first; in a large image, regions above 32 KiB
+// are faster as one launch per level).  Its LDS holds C, P and Q: 1.3125
+// regions at most.  It reads its whole region before writing, so it works in
+// place.
+static long long g_block_bytes = 32 * 1024;
+constexpr int kBlockLdsMax = 128 * 1024;
+constexpr long long kBlockRegionMax = 48 * 1024;  // 2 x 48 KiB x 1.3125 < kBlockLdsMax
 
 void tail_config(long long max_region_bytes, int) {
-  g_block_bytes = max_region_bytes > kBlockLdsMax / 2 ? kBlockLdsMax / 2 : max_region_bytes;
+  g_block_bytes = max_region_bytes > kBlockRegionMax ? kBlockRegionMax : max_region_bytes;
+}
+
+static long long block_lds_bytes(long long W0, long long H0, int elem) {
+  return elem * (W0 * H0 + ((W0 + 1) / 2) * ((H0 + 1) / 2) + ((W0 + 3) / 4) * ((H0 + 3) / 4));
 }
 
 int tail_start(int W, int H, int L, int elem) {
   if (g_block_bytes <= 0) return L;
   for (int j = 0; j < L; ++j) {
     const long long wj = ceil_div_pow2(W, j), hj = ceil_div_pow2(H, j);
-    if (wj * hj * elem <= g_block_bytes) return j;
+    const long long limit = j == 0 ? 2 * g_block_bytes : g_block_bytes;
+    if (wj * hj * elem <= limit && block_lds_bytes(wj, hj, elem) <= kBlockLdsMax) return j;
   }
   return L;
 }
 
 hipError_t configure_tail_kernels() {
-  const void* fns[] = {reinterpret_cast<const void*>(&block_fwd_kernel<float>),
-                       reinterpret_cast<const void*>(&block_fwd_kernel<double>),
-                       reinterpret_cast<const void*>(&block_inv_kernel<float>),
-                       reinterpret_cast<const void*>(&block_inv_kernel<double>)};
+  const void* fns[] = {reinterpret_cast<const void*>(&block_fwd_kernel<float, 4>),
+                       reinterpret_cast<const void*>(&block_fwd_kernel<float, 2>),
+                       reinterpret_cast<const void*>(&block_fwd_kernel<double, 4>),
+                       reinterpret_cast<const void*>(&block_fwd_kernel<double, 2>),
+                       reinterpret_cast<const void*>(&block_inv_kernel<float, 4>),
+                       reinterpret_cast<const void*>(&block_inv_kernel<float, 2>),
+                       reinterpret_cast<const void*>(&block_inv_kernel<double, 4>),
+                       reinterpret_cast<const void*>(&block_inv_kernel<double, 2>)};
   for (const void* f : fns) {
     const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kBlockLdsMax);
     if (e != hipSuccess) {
@@ -370,17 +330,25 @@ hipError_t configure_tail_kernels() {
 template <typename T>
 void launch_block_fwd(const T* src, long long sp, T* dst, long long dp, int W0, int H0, int nlev,
                       const Coeffs<T>& cf, hipStream_t st) {
-  const size_t lds = 2 * (size_t)W0 * H0 * sizeof(T);
-  hipLaunchKernelGGL((block_fwd_kernel<T>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst, dp,
-                     W0, H0, nlev, cf);
+  const size_t lds = (size_t)block_lds_bytes(W0, H0, (int)sizeof(T));
+  if (cf.nsteps == 4)
+    hipLaunchKernelGGL((block_fwd_kernel<T, 4>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst,
+                       dp, W0, H0, nlev, cf);
+  else
+    hipLaunchKernelGGL((block_fwd_kernel<T, 2>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst,
+                       dp, W0, H0, nlev, cf);
 }
 
 template <typename T>
 void launch_block_inv(const T* src, long long sp, T* dst, long long dp, int W0, int H0, int nlev,
                       const Coeffs<T>& cf, hipStream_t st) {
-  const size_t lds = 2 * (size_t)W0 * H0 * sizeof(T);
-  hipLaunchKernelGGL((block_inv_kernel<T>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst, dp,
-                     W0, H0, nlev, cf);
+  const size_t lds = (size_t)block_lds_bytes(W0, H0, (int)sizeof(T));
+  if (cf.nsteps == 4)
+    hipLaunchKernelGGL((block_inv_kernel<T, 4>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst,
+                       dp, W0, H0, nlev, cf);
+  else
+    hipLaunchKernelGGL((block_inv_kernel<T, 2>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst,
+                       dp, W0, H0, nlev, cf);
 }
 
 template void launch_block_fwd<float>(const float*, long long, float*, long long, int, int, int,

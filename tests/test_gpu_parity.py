@@ -171,6 +171,40 @@ def test_async_api_on_torch_stream(gold):
     np.testing.assert_array_equal(t.cpu().numpy(), want)
 
 
+BLOCK_SHAPES = [(128, 128), (127, 129), (255, 64), (100, 163), (2, 2), (3, 17), (33, 9),
+                (1500, 5), (7, 2000), (90, 91)]
+
+
+@pytest.mark.parametrize("prec", ["s", "d"])
+def test_block_kernel_whole_images(prec):
+    """Images the single-workgroup LDS kernel transforms completely (and at
+    the larger threshold images it takes over after level 0 or 1), bit-exact
+    to the oracle, forward and inverse, both wavelets, full depth and J=2."""
+    from oracle.oracle import oracle
+    o = oracle()
+    lib = libdwt_amd.lib()
+    dt = np.float32 if prec == "s" else np.float64
+    rng = np.random.default_rng(5)
+    try:
+        for tb in (48 * 1024, 4 * 1024):
+            assert lib.dwt_gpu_tail_set(tb) == 0
+            for (W, H) in BLOCK_SHAPES:
+                img = rng.standard_normal((H, W)).astype(dt)
+                for w in cases.WAVELETS:
+                    for J in (-1, 2):
+                        say(f"block {W}x{H} {w} {prec} J={J} threshold {tb}")
+                        ref, jr = o.fwd2(img, w, J)
+                        t = torch.from_numpy(img).cuda()
+                        j = libdwt_amd.fwd2(t, w, J)
+                        assert j == jr
+                        np.testing.assert_array_equal(t.cpu().numpy(), ref, err_msg="forward")
+                        libdwt_amd.inv2(t, w, j)
+                        np.testing.assert_array_equal(t.cpu().numpy(), o.inv2(ref, w, j),
+                                                      err_msg="inverse")
+    finally:
+        lib.dwt_gpu_tail_set(32 * 1024)
+
+
 PYR_SHAPES = [(2, 2), (3, 5), (17, 33), (130, 70), (300, 257), (1000, 40), (9, 250), (513, 511),
               (2048, 3)]
 
@@ -206,7 +240,7 @@ def test_pyramid_schedule_shapes(prec):
                     libdwt_amd.device_check(st)
                     np.testing.assert_array_equal(t.cpu().numpy(), want, err_msg=f"inverse {sched}")
     finally:
-        lib.dwt_gpu_tail_set(16 * 1024)
+        lib.dwt_gpu_tail_set(32 * 1024)
         libdwt_amd.schedule_set(0)
 
 
