@@ -39,16 +39,17 @@ __device__ inline int xcd_remap(int b, int nb) {
   return xcd * per + (xcd < rem ? xcd : rem) + idx;
 }
 
-// Value held by lane+1 / lane-1 of the wavefront, as a DPP modifier on a
-// VALU move (wave_shl:1 / wave_shr:1) rather than an LDS-crossbar permute, so
-// a lifting step's neighbour costs a few cycles instead of an LDS round trip.
-// The edge lanes (0 and 63) receive their own value; they are halo lanes whose
-// results are never stored.
+// Value held by lane+1 / lane-1 of the wavefront, as a DPP modifier
+// (wave_shl:1 / wave_shr:1) rather than an LDS-crossbar permute.  With the
+// out-of-range lane reading zero (bound_ctrl) and full row / bank masks, the
+// compiler folds the move into the add that consumes it (v_add_f32_dpp), so a
+// lifting step's neighbour costs no instruction of its own.  The edge lanes
+// (0 and 63) receive 0; they are halo lanes whose results are never stored.
 __device__ inline int dpp_from_next(int v) {
-  return __builtin_amdgcn_update_dpp(v, v, 0x130, 0xF, 0xF, false);  // wave_shl:1
+  return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, true);  // wave_shl:1
 }
 __device__ inline int dpp_from_prev(int v) {
-  return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xF, 0xF, false);  // wave_shr:1
+  return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, true);  // wave_shr:1
 }
 __device__ inline float from_next_lane(float v) {
   return __int_as_float(dpp_from_next(__float_as_int(v)));
