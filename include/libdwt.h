@@ -265,7 +265,8 @@ int dwt_gpu_profile_reset(void);
 
 /* Tile shape of the packed level kernels: a wavefront walks seg_rows rows
  * (even, 2..4096) of a 128-column strip, prefetching prefetch_pairs (4 or 8)
- * row pairs.  Shapes are kept per direction (forward 1 / inverse 0) and level
+ * row pairs, or (prefetch_pairs 0) requests all rows of its segment at once
+ * (seg_rows 8, 12, 16, 20, 24 or 32).  Shapes are kept per direction (forward 1 / inverse 0) and level
  * height class ceil(log2(height)); seg_rows = 0 restores the built-in choice.
  * dwt_gpu_tile_get reports the shape a level of that size would use. */
 int dwt_gpu_tile_set(int forward, int height_class, int seg_rows, int prefetch_pairs);

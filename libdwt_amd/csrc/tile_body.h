@@ -230,15 +230,15 @@ constexpr int kHL = 0, kLH = 1, kHH = 2;
 // addresses memory through a pointer of one known address space.
 //
 // LoadRow(Y, Xe, xe, xo, interior, e, o): samples (Xe, Xe + 1) of row Y of the
-//   level input, Y possibly outside [0, H) (the policy mirrors it); xe / xo are
-//   the mirrored column indices, `interior` says the wave's whole window lies
-//   inside the row.
+//   level input, Y in [0, H) (the tile body mirrors rows beyond the ends); xe /
+//   xo are the mirrored column indices, `interior` says the wave's whole
+//   window lies inside the row.
 // StoreLL(m, k, v): LL coefficient (row m, column k) of the level output.
 // Det.store(band, r, c, v) / Det.load(band, r, c): a detail coefficient at row
 //   r, column c of the level's Mallat layout (band: kHL, kLH or kHH).
 template <typename T, int NS, int U, bool INTERIOR, class LoadRow, class StoreLL, class Det>
 __device__ __forceinline__ void fwd_tile_impl(int strip, int seg, int S, int W, int H,
-                                              const LoadRow& load_row, const StoreLL& store_ll,
+                                              const LoadRow& load_row_in, const StoreLL& store_ll,
                                               const Det& det, const Coeffs<T>& cf) {
   constexpr int HALO = NS;
   constexpr int CORE = 2 * kWave - 2 * HALO;
@@ -258,6 +258,10 @@ __device__ __forceinline__ void fwd_tile_impl(int strip, int seg, int S, int W, 
   const int m1 = min((seg * S + S) >> 1, Hd);
   const int pbeg = m0 - LAG, pend = m1 + LAG - 1;
 
+  // INTERIOR: every row and column the tile reads lies inside the level
+  auto load_row = [&](int Y, int Xe_, int xe_, int xo_, bool in, T& e, T& o) {
+    load_row_in(INTERIOR ? Y : mirror_near(Y, H), Xe_, xe_, xo_, in, e, o);
+  };
   ColFwd<T, NS> ca, cb;
   auto pair = [&](int p, T e0, T o0, T e1, T o1) {
     row_fwd<T, NS>(e0, o0, cf);
@@ -308,17 +312,41 @@ __device__ __forceinline__ void fwd_tile_impl(int strip, int seg, int S, int W, 
   }
 }
 
+// Rows [first, last] that a tile of segment `seg` reads (before mirroring):
+// the segment's row pairs plus LAG pairs of halo on each side, and for the
+// rolling mode the prefetch of up to 2U more pairs past the end.
+template <int NS, int U>
+__device__ __forceinline__ bool rows_inside(int seg, int S, int H, bool fwd) {
+  constexpr int LAG = NS / 2;
+  int first, last;
+  if (fwd) {
+    const int m0 = (seg * S) >> 1, m1 = min((seg * S + S) >> 1, (H + 1) >> 1);
+    const int pbeg = m0 - LAG, pend = m1 + LAG - 1;
+    first = 2 * pbeg;
+    last = U < 0 ? 2 * (pbeg + (-U / 2 + NS)) - 1 : 2 * (pend + 2 * U) - 1;
+  } else {
+    const int y0 = seg * S, y1 = min(y0 + S, H);
+    const int pbeg = (y0 >> 1) - LAG, pend = (y1 >> 1) + LAG + 1;
+    first = 2 * pbeg;
+    last = U < 0 ? 2 * (pbeg + (-U / 2 + NS)) - 1 : 2 * (pend + 2 * U) - 1;
+  }
+  return first >= 0 && last < H;
+}
+
 // The whole 128-column window of a strip either lies inside the rows (plain,
-// vectorisable loads) or touches an end (mirrored columns).  The test is the
-// same for every lane of the wavefront; readfirstlane makes it a scalar branch
-// between two separately compiled bodies.
+// vectorisable loads) or touches an end (mirrored columns); likewise the
+// tile's rows.  The test is the same for every lane of the wavefront;
+// readfirstlane makes it a scalar branch between two separately compiled
+// bodies, so interior tiles (almost all of a large level) spend no
+// instructions on mirroring.
 template <typename T, int NS, int U, class LoadRow, class StoreLL, class Det>
 __device__ __forceinline__ void fwd_tile(int strip, int seg, int S, int W, int H,
                                          const LoadRow& load_row, const StoreLL& store_ll,
                                          const Det& det, const Coeffs<T>& cf) {
   constexpr int HALO = NS;
   const int x0 = strip * (2 * kWave - 2 * HALO);
-  const int interior = __builtin_amdgcn_readfirstlane((x0 - HALO >= 0) && (x0 - HALO + 2 * kWave <= W));
+  const int interior = __builtin_amdgcn_readfirstlane((x0 - HALO >= 0) && (x0 - HALO + 2 * kWave <= W) &&
+                                                      rows_inside<NS, U>(seg, S, H, true));
   if (interior)
     fwd_tile_impl<T, NS, U, true>(strip, seg, S, W, H, load_row, store_ll, det, cf);
   else
@@ -329,10 +357,10 @@ __device__ __forceinline__ void fwd_tile(int strip, int seg, int S, int W, int H
 // W x H fine region.  LoadLL(m, k): LL coefficient; details come from det at
 // their Mallat positions.  StoreOut(r, Xe, e, o, st_e, st_o): output samples
 // (Xe, Xe + 1) of row r.
-template <typename T, int NS, int U, class LoadLL, class Det, class StoreOut>
-__device__ __forceinline__ void inv_tile(int strip, int seg, int S, int W, int H,
-                                         const LoadLL& load_ll, const Det& det,
-                                         const StoreOut& store_out, const Coeffs<T>& cf) {
+template <typename T, int NS, int U, bool ROWS_IN, class LoadLL, class Det, class StoreOut>
+__device__ __forceinline__ void inv_tile_impl(int strip, int seg, int S, int W, int H,
+                                              const LoadLL& load_ll, const Det& det,
+                                              const StoreOut& store_out, const Coeffs<T>& cf) {
   constexpr int HALO = NS;
   constexpr int CORE = 2 * kWave - 2 * HALO;
   constexpr int LAG = NS / 2;
@@ -350,7 +378,7 @@ __device__ __forceinline__ void inv_tile(int strip, int seg, int S, int W, int H
   const int pbeg = (y0 >> 1) - LAG, pend = (y1 >> 1) + LAG + 1;
 
   auto load_row = [&](int Y, T& e, T& o) {
-    const int Yr = mirror_near(Y, H);
+    const int Yr = ROWS_IN ? Y : mirror_near(Y, H);
     const int m = Yr >> 1;
     if (Yr & 1) {
       e = det.load(kLH, Hc + m, ke);
@@ -401,6 +429,17 @@ __device__ __forceinline__ void inv_tile(int strip, int seg, int S, int W, int H
   }
 }
 
+// Rows of the tile inside the level (almost every tile): no mirroring.
+template <typename T, int NS, int U, class LoadLL, class Det, class StoreOut>
+__device__ __forceinline__ void inv_tile(int strip, int seg, int S, int W, int H,
+                                         const LoadLL& load_ll, const Det& det,
+                                         const StoreOut& store_out, const Coeffs<T>& cf) {
+  if (__builtin_amdgcn_readfirstlane(rows_inside<NS, U>(seg, S, H, false)))
+    inv_tile_impl<T, NS, U, true>(strip, seg, S, W, H, load_ll, det, store_out, cf);
+  else
+    inv_tile_impl<T, NS, U, false>(strip, seg, S, W, H, load_ll, det, store_out, cf);
+}
+
 // ---- row sources and destinations ------------------------------------------------
 template <typename T, bool VEC>
 struct GlobalRows {  // packed rows in global memory
@@ -410,7 +449,7 @@ struct GlobalRows {  // packed rows in global memory
   __device__ __forceinline__ void operator()(int Y, int Xe, int xe, int xo, bool interior, T& e,
                                              T& o) const {
     using V2 = typename Vec2<T>::type;
-    const Gbl<T>* rp = gload_ptr(p) + (long long)mirror_near(Y, H) * pitch;
+    const Gbl<T>* rp = gload_ptr(p) + (long long)Y * pitch;
     if (VEC && interior) {
       const V2 v = *(const Gbl<V2>*)(rp + Xe);
       e = v.x;
@@ -433,7 +472,7 @@ struct LdsRows {  // packed rows in LDS (pitch = W)
   const Lds<T>* p;
   int pitch, H;
   __device__ __forceinline__ void operator()(int Y, int, int xe, int xo, bool, T& e, T& o) const {
-    const Lds<T>* rp = p + mirror_near(Y, H) * pitch;
+    const Lds<T>* rp = p + Y * pitch;
     e = rp[xe];
     o = rp[xo];
   }

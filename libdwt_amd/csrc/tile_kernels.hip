@@ -15,7 +15,8 @@ fwd_level_kernel(const T* __restrict__ src, long long sp, int W, int H, T* __res
                  long long llp, T* __restrict__ det, long long dp, int S, int nstrips, int nseg,
                  Coeffs<T> cf) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int g = b / nseg, s = b - g * nseg;
+  const int ngroups = (nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int s = b / ngroups, g = b - s * ngroups;
   const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
   fwd_tile<T, NS, U>(strip, s, S, W, H, GlobalRows<T, VEC>{src, sp, H}, GlobalStore<T>{ll, llp},
@@ -28,7 +29,8 @@ inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ 
                  T* __restrict__ dst, long long dstp, int W, int H, int S, int nstrips, int nseg,
                  Coeffs<T> cf) {
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int g = b / nseg, s = b - g * nseg;
+  const int ngroups = (nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int s = b / ngroups, g = b - s * ngroups;
   const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
   inv_tile<T, NS, U>(strip, s, S, W, H, GlobalLoad<T>{ll, llp},
@@ -184,8 +186,9 @@ int tile_set(int fwd, int hclass, int S, int U) {
   if (hclass < 0 || hclass >= 32) return -1;
   if (S != 0 && (S < 2 || (S & 1) || S > 4096)) return -1;
   // U = 4 or 8: rolling prefetch of U row pairs; U = 0: single shot, the
-  // whole segment requested at once (S = 8, 16 or 32)
-  if (S != 0 && U != 4 && U != 8 && !(U == 0 && (S == 8 || S == 16 || S == 32))) return -1;
+  // whole segment requested at once (S = 8, 12, 16, 20, 24 or 32)
+  const bool single = U == 0 && (S == 8 || S == 12 || S == 16 || S == 20 || S == 24 || S == 32);
+  if (S != 0 && U != 4 && U != 8 && !single) return -1;
   g_tiles[fwd ? 0 : 1][hclass] = TileShape{S, U};
   return 0;
 }
@@ -204,13 +207,18 @@ static bool aligned_pair(const void* p, long long pitch) {
 }
 
 // Instantiation for a tile shape: rolling prefetch (U = 4 / 8, any S) or
-// single shot (U = 0, S in {8, 16, 32}, template U = -S).
+// single shot (U = 0, S in {8, 12, 16, 20, 24, 32}, template U = -S).
 template <typename T, int NS, bool VEC, class Launch>
 static void with_shape(const TileShape& t, const Launch& launch) {
   if (t.U == 0) {
-    if (t.S == 8) launch.template go<T, NS, VEC, -8>();
-    else if (t.S == 16) launch.template go<T, NS, VEC, -16>();
-    else launch.template go<T, NS, VEC, -32>();
+    switch (t.S) {
+      case 8: launch.template go<T, NS, VEC, -8>(); break;
+      case 12: launch.template go<T, NS, VEC, -12>(); break;
+      case 16: launch.template go<T, NS, VEC, -16>(); break;
+      case 20: launch.template go<T, NS, VEC, -20>(); break;
+      case 24: launch.template go<T, NS, VEC, -24>(); break;
+      default: launch.template go<T, NS, VEC, -32>(); break;
+    }
   } else if (t.U == 8) {
     launch.template go<T, NS, VEC, 8>();
   } else {
