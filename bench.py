@@ -525,7 +525,8 @@ def main():
     libdwt_amd.profile_enable(False)
     kernels = {}
     for kind, name in ((0, "fwd_level"), (1, "inv_level"), (2, "detail_copy"), (4, "fwd_block"),
-                       (5, "inv_block"), (6, "fwd_pyramid"), (7, "inv_pyramid")):
+                       (5, "inv_block"), (6, "fwd_pyramid"), (7, "inv_pyramid"), (10, "fwd_mid"),
+                       (11, "inv_mid")):
         for lev in range(J):
             n, ms = libdwt_amd.profile_read(kind, lev)
             if n:

@@ -266,7 +266,7 @@ int dwt_gpu_profile_reset(void);
 /* Tile shape of the packed level kernels: a wavefront walks seg_rows rows
  * (even, 2..4096) of a 128-column strip, prefetching prefetch_pairs (4 or 8)
  * row pairs, or (prefetch_pairs 0) requests all rows of its segment at once
- * (seg_rows 8, 12, 16, 20, 24 or 32).  Shapes are kept per direction (forward 1 / inverse 0) and level
+ * (seg_rows 2, 4, 8, 12, 16, 20, 24 or 32).  Shapes are kept per direction (forward 1 / inverse 0) and level
  * height class ceil(log2(height)); seg_rows = 0 restores the built-in choice.
  * dwt_gpu_tile_get reports the shape a level of that size would use. */
 int dwt_gpu_tile_set(int forward, int height_class, int seg_rows, int prefetch_pairs);
@@ -279,6 +279,13 @@ int dwt_gpu_tile_get(int forward, int width, int height, int wavelet, int *seg_r
  * disables it).  An image of at most twice that is transformed completely by
  * that one launch. */
 int dwt_gpu_tail_set(long long max_region_bytes);
+
+/* Optionally, three middle levels of the packed path run as one launch (each
+ * workgroup computes overlapping windows of the three levels in LDS) from the
+ * first level j >= 1 whose region is at most max_level_bytes, when all three
+ * come before the single-workgroup launch.  Default 0 (off): measured slower
+ * than one launch per level on MI355X. */
+int dwt_gpu_mid_set(long long max_level_bytes);
 
 /* Schedule of the levels after level 0 on the packed path: 0 (default) runs
  * one launch per level plus a single-workgroup launch for the small levels;

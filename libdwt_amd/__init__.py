@@ -95,6 +95,8 @@ def lib() -> C.CDLL:
         L.dwt_gpu_tile_get.restype = i
         L.dwt_gpu_tail_set.argtypes = [C.c_longlong]
         L.dwt_gpu_tail_set.restype = i
+        L.dwt_gpu_mid_set.argtypes = [C.c_longlong]
+        L.dwt_gpu_mid_set.restype = i
         L.dwt_gpu_scratch_bytes.argtypes = [vp]
         L.dwt_gpu_scratch_bytes.restype = C.c_longlong
         L.dwt_gpu_release_scratch.argtypes = [vp]

@@ -17,9 +17,9 @@ import math
 import libdwt_amd
 
 # (S, U): U = 4 / 8 rolling prefetch with any S; U = 0 single shot (the whole
-# segment requested at once), S in 8, 12, 16, 20, 24, 32
+# segment requested at once), S in 2, 4, 8, 12, 16, 20, 24, 32
 CANDIDATES = ([(S, 4) for S in (8, 16, 32, 64)] + [(S, 8) for S in (16, 32)]
-              + [(S, 0) for S in (8, 12, 16, 20, 24, 32)])
+              + [(S, 0) for S in (2, 4, 8, 12, 16, 20, 24, 32)])
 
 
 def height_class(h: int) -> int:

@@ -53,8 +53,9 @@ int level_limit(int Wo, int Ho, int d1);
 // Optional per-launch timing with HIP events on the launch stream.  Kinds:
 // 0 forward level kernel, 1 inverse level kernel, 2 detail copy, 4 forward
 // block kernel, 5 inverse block kernel, 6 forward pyramid, 7 inverse pyramid,
-// 8 / 9 forward / inverse batched level launch.  Level: the level index the
-// launch belongs to (the first level it does for a block kernel or pyramid).
+// 8 / 9 forward / inverse batched level launch, 10 / 11 forward / inverse
+// fused middle levels.  Level: the level index the launch belongs to (the
+// first level it does for a block kernel, pyramid or fused launch).
 struct ProfRec {
   int kind, level;
   hipEvent_t a, b;

@@ -50,6 +50,20 @@ void launch_block_inv(const T* src, long long sp, T* dst, long long dp, int W0, 
 int tail_start(int W, int H, int L, int elem);
 void tail_config(long long max_region_bytes, int unused);
 hipError_t configure_tail_kernels();
+// Fused middle levels (mid_kernels.hip): nl (<= kMidMaxLevels) consecutive
+// levels in one launch, each workgroup computing overlapping LDS windows.
+// Forward: LL_j (W0 x H0) from src; details of the nl levels into det (Mallat
+// layout, in place); LL_{j+nl} to ll.  Inverse: LL_{j+nl} from llsrc,
+// details from det; LL_j (W0 x H0) to out.
+constexpr int kMidMaxLevels = 4;
+template <typename T>
+void launch_mid_fwd(const T* src, long long sp, T* det, long long dp, T* ll, long long llp, int W0,
+                    int H0, int nl, const Coeffs<T>& cf, hipStream_t st);
+template <typename T>
+void launch_mid_inv(const T* llsrc, long long lp, const T* det, long long dp, T* out, long long op,
+                    int W0, int H0, int nl, const Coeffs<T>& cf, hipStream_t st);
+size_t mid_lds_bytes(bool fwd, int nl, int nsteps, int elem);
+hipError_t configure_mid_kernels();
 // Copy a w x h rectangle between packed buffers.
 template <typename T>
 void launch_copy_rect(const T* src, long long sp, T* dst, long long dp, int w, int h,

@@ -19,10 +19,17 @@
 // the image are read from their mirror images, which reproduces the
 // reference's end-of-line update  x + (2c) * n  bit for bit.
 //
-// Four waves of a workgroup take four adjacent strips of the same segment,
-// and the block index is remapped so that blocks resident on one XCD walk
-// down neighbouring segments (their halo rows then hit that XCD's L2).
+// Four waves of a workgroup take four adjacent strips of the same segment.
+// Tiles are numbered row-major and the block index is remapped so that each
+// XCD takes a contiguous run of them: a horizontal band of the level, read
+// row after row, whose halo rows and columns hit that XCD's L2.
+//
+// A tile's strips and segments can start at an offset (xoff, yoff: even) so
+// that a fused multi-level kernel computes a window of a level rather than
+// the whole of it; every index stays in level coordinates.
 #pragma once
+
+#include <type_traits>
 
 #include "dwt_common.h"
 
@@ -239,12 +246,12 @@ constexpr int kHL = 0, kLH = 1, kHH = 2;
 template <typename T, int NS, int U, bool INTERIOR, class LoadRow, class StoreLL, class Det>
 __device__ __forceinline__ void fwd_tile_impl(int strip, int seg, int S, int W, int H,
                                               const LoadRow& load_row_in, const StoreLL& store_ll,
-                                              const Det& det, const Coeffs<T>& cf) {
+                                              const Det& det, const Coeffs<T>& cf, int xoff, int yoff) {
   constexpr int HALO = NS;
   constexpr int CORE = 2 * kWave - 2 * HALO;
   constexpr int LAG = NS / 2;
   const int lane = threadIdx.x & (kWave - 1);
-  const int x0 = strip * CORE;
+  const int x0 = xoff + strip * CORE;
   const int Xe = x0 - HALO + 2 * lane;
   const int xe = INTERIOR ? Xe : mirror(Xe, W), xo = INTERIOR ? Xe + 1 : mirror(Xe + 1, W);
   const int Wd = (W + 1) >> 1, Wh = W >> 1, Hd = (H + 1) >> 1, Hh = H >> 1;
@@ -254,8 +261,8 @@ __device__ __forceinline__ void fwd_tile_impl(int strip, int seg, int S, int W, 
   const bool st_b = core && k < Wh;
   constexpr bool interior = INTERIOR;
 
-  const int m0 = (seg * S) >> 1;
-  const int m1 = min((seg * S + S) >> 1, Hd);
+  const int m0 = (yoff + seg * S) >> 1;
+  const int m1 = min((yoff + seg * S + S) >> 1, Hd);
   const int pbeg = m0 - LAG, pend = m1 + LAG - 1;
 
   // INTERIOR: every row and column the tile reads lies inside the level
@@ -312,20 +319,21 @@ __device__ __forceinline__ void fwd_tile_impl(int strip, int seg, int S, int W, 
   }
 }
 
-// Rows [first, last] that a tile of segment `seg` reads (before mirroring):
+// Rows [first, last] that a tile whose segment starts at row ys reads
+// (before mirroring):
 // the segment's row pairs plus LAG pairs of halo on each side, and for the
 // rolling mode the prefetch of up to 2U more pairs past the end.
 template <int NS, int U>
-__device__ __forceinline__ bool rows_inside(int seg, int S, int H, bool fwd) {
+__device__ __forceinline__ bool rows_inside(int ys, int S, int H, bool fwd) {
   constexpr int LAG = NS / 2;
   int first, last;
   if (fwd) {
-    const int m0 = (seg * S) >> 1, m1 = min((seg * S + S) >> 1, (H + 1) >> 1);
+    const int m0 = ys >> 1, m1 = min((ys + S) >> 1, (H + 1) >> 1);
     const int pbeg = m0 - LAG, pend = m1 + LAG - 1;
     first = 2 * pbeg;
     last = U < 0 ? 2 * (pbeg + (-U / 2 + NS)) - 1 : 2 * (pend + 2 * U) - 1;
   } else {
-    const int y0 = seg * S, y1 = min(y0 + S, H);
+    const int y0 = ys, y1 = min(y0 + S, H);
     const int pbeg = (y0 >> 1) - LAG, pend = (y1 >> 1) + LAG + 1;
     first = 2 * pbeg;
     last = U < 0 ? 2 * (pbeg + (-U / 2 + NS)) - 1 : 2 * (pend + 2 * U) - 1;
@@ -342,15 +350,15 @@ __device__ __forceinline__ bool rows_inside(int seg, int S, int H, bool fwd) {
 template <typename T, int NS, int U, class LoadRow, class StoreLL, class Det>
 __device__ __forceinline__ void fwd_tile(int strip, int seg, int S, int W, int H,
                                          const LoadRow& load_row, const StoreLL& store_ll,
-                                         const Det& det, const Coeffs<T>& cf) {
+                                         const Det& det, const Coeffs<T>& cf, int xoff = 0, int yoff = 0) {
   constexpr int HALO = NS;
-  const int x0 = strip * (2 * kWave - 2 * HALO);
+  const int x0 = xoff + strip * (2 * kWave - 2 * HALO);
   const int interior = __builtin_amdgcn_readfirstlane((x0 - HALO >= 0) && (x0 - HALO + 2 * kWave <= W) &&
-                                                      rows_inside<NS, U>(seg, S, H, true));
+                                                      rows_inside<NS, U>(yoff + seg * S, S, H, true));
   if (interior)
-    fwd_tile_impl<T, NS, U, true>(strip, seg, S, W, H, load_row, store_ll, det, cf);
+    fwd_tile_impl<T, NS, U, true>(strip, seg, S, W, H, load_row, store_ll, det, cf, xoff, yoff);
   else
-    fwd_tile_impl<T, NS, U, false>(strip, seg, S, W, H, load_row, store_ll, det, cf);
+    fwd_tile_impl<T, NS, U, false>(strip, seg, S, W, H, load_row, store_ll, det, cf, xoff, yoff);
 }
 
 // Inverse tile: output columns of strip `strip`, rows of segment `seg` of the
@@ -360,12 +368,12 @@ __device__ __forceinline__ void fwd_tile(int strip, int seg, int S, int W, int H
 template <typename T, int NS, int U, bool ROWS_IN, class LoadLL, class Det, class StoreOut>
 __device__ __forceinline__ void inv_tile_impl(int strip, int seg, int S, int W, int H,
                                               const LoadLL& load_ll, const Det& det,
-                                              const StoreOut& store_out, const Coeffs<T>& cf) {
+                                              const StoreOut& store_out, const Coeffs<T>& cf, int xoff, int yoff) {
   constexpr int HALO = NS;
   constexpr int CORE = 2 * kWave - 2 * HALO;
   constexpr int LAG = NS / 2;
   const int lane = threadIdx.x & (kWave - 1);
-  const int x0 = strip * CORE;
+  const int x0 = xoff + strip * CORE;
   const int Xe = x0 - HALO + 2 * lane;
   const int ke = mirror(Xe, W) >> 1, ko = mirror(Xe + 1, W) >> 1;
   const int Wc = (W + 1) >> 1, Hc = (H + 1) >> 1;
@@ -373,7 +381,7 @@ __device__ __forceinline__ void inv_tile_impl(int strip, int seg, int S, int W, 
   const bool st_e = core && Xe < W;
   const bool st_o = core && Xe + 1 < W;
 
-  const int y0 = seg * S;
+  const int y0 = yoff + seg * S;
   const int y1 = min(y0 + S, H);
   const int pbeg = (y0 >> 1) - LAG, pend = (y1 >> 1) + LAG + 1;
 
@@ -433,11 +441,50 @@ __device__ __forceinline__ void inv_tile_impl(int strip, int seg, int S, int W, 
 template <typename T, int NS, int U, class LoadLL, class Det, class StoreOut>
 __device__ __forceinline__ void inv_tile(int strip, int seg, int S, int W, int H,
                                          const LoadLL& load_ll, const Det& det,
-                                         const StoreOut& store_out, const Coeffs<T>& cf) {
-  if (__builtin_amdgcn_readfirstlane(rows_inside<NS, U>(seg, S, H, false)))
-    inv_tile_impl<T, NS, U, true>(strip, seg, S, W, H, load_ll, det, store_out, cf);
+                                         const StoreOut& store_out, const Coeffs<T>& cf, int xoff = 0,
+                                         int yoff = 0) {
+  if (__builtin_amdgcn_readfirstlane(rows_inside<NS, U>(yoff + seg * S, S, H, false)))
+    inv_tile_impl<T, NS, U, true>(strip, seg, S, W, H, load_ll, det, store_out, cf, xoff, yoff);
   else
-    inv_tile_impl<T, NS, U, false>(strip, seg, S, W, H, load_ll, det, store_out, cf);
+    inv_tile_impl<T, NS, U, false>(strip, seg, S, W, H, load_ll, det, store_out, cf, xoff, yoff);
+}
+
+// ---- small levels: tile height by latency ------------------------------------
+// A level of a few tiles is latency-bound: its time is the number of tile
+// rounds a wavefront runs times the row pairs of one tile (S/2 + NS, the
+// tile's serial chain).  Short segments give more, shorter tiles; pick the
+// height in {8, 4, 2} with the shortest chain for nw wavefronts, then run
+// f(std::integral_constant<int, S>, strip, seg) for this wavefront's tiles.
+template <int NS>
+__device__ __forceinline__ int small_rows(int nstrips, int rows, int nw) {
+  int best = 8, cost = 1 << 30;
+  for (int S = 8; S >= 2; S >>= 1) {
+    const int nt = nstrips * ((rows + S - 1) / S);
+    const int c = ((nt + nw - 1) / nw) * (S / 2 + NS);
+    if (c < cost) {
+      cost = c;
+      best = S;
+    }
+  }
+  return best;
+}
+template <int NS, class F>
+__device__ __forceinline__ void for_small_tiles(int nstrips, int rows, int wave, int nw, const F& f) {
+  const int S = __builtin_amdgcn_readfirstlane(small_rows<NS>(nstrips, rows, nw));
+  auto run = [&](auto sc) {
+    constexpr int SR = decltype(sc)::value;
+    const int ntiles = nstrips * ((rows + SR - 1) / SR);
+    for (int t = wave; t < ntiles; t += nw) {
+      const int seg = t / nstrips;
+      f(sc, t - seg * nstrips, seg);
+    }
+  };
+  if (S == 8)
+    run(std::integral_constant<int, 8>{});
+  else if (S == 4)
+    run(std::integral_constant<int, 4>{});
+  else
+    run(std::integral_constant<int, 2>{});
 }
 
 // ---- row sources and destinations ------------------------------------------------

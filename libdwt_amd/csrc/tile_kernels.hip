@@ -186,8 +186,9 @@ int tile_set(int fwd, int hclass, int S, int U) {
   if (hclass < 0 || hclass >= 32) return -1;
   if (S != 0 && (S < 2 || (S & 1) || S > 4096)) return -1;
   // U = 4 or 8: rolling prefetch of U row pairs; U = 0: single shot, the
-  // whole segment requested at once (S = 8, 12, 16, 20, 24 or 32)
-  const bool single = U == 0 && (S == 8 || S == 12 || S == 16 || S == 20 || S == 24 || S == 32);
+  // whole segment requested at once (S = 2, 4, 8, 12, 16, 20, 24 or 32)
+  const bool single = U == 0 && (S == 2 || S == 4 || S == 8 || S == 12 || S == 16 || S == 20 ||
+                                 S == 24 || S == 32);
   if (S != 0 && U != 4 && U != 8 && !single) return -1;
   g_tiles[fwd ? 0 : 1][hclass] = TileShape{S, U};
   return 0;
@@ -207,11 +208,13 @@ static bool aligned_pair(const void* p, long long pitch) {
 }
 
 // Instantiation for a tile shape: rolling prefetch (U = 4 / 8, any S) or
-// single shot (U = 0, S in {8, 12, 16, 20, 24, 32}, template U = -S).
+// single shot (U = 0, S in {2, 4, 8, 12, 16, 20, 24, 32}, template U = -S).
 template <typename T, int NS, bool VEC, class Launch>
 static void with_shape(const TileShape& t, const Launch& launch) {
   if (t.U == 0) {
     switch (t.S) {
+      case 2: launch.template go<T, NS, VEC, -2>(); break;
+      case 4: launch.template go<T, NS, VEC, -4>(); break;
       case 8: launch.template go<T, NS, VEC, -8>(); break;
       case 12: launch.template go<T, NS, VEC, -12>(); break;
       case 16: launch.template go<T, NS, VEC, -16>(); break;

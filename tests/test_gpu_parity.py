@@ -205,6 +205,43 @@ def test_block_kernel_whole_images(prec):
         lib.dwt_gpu_tail_set(32 * 1024)
 
 
+MID_SHAPES = [(1000, 700), (513, 511), (777, 1500), (257, 4000), (4000, 130), (96, 96), (300, 257)]
+
+
+@pytest.mark.parametrize("prec", ["s", "d"])
+def test_mid_kernel_shapes(prec):
+    """Three middle levels fused into one launch (each workgroup computes
+    overlapping windows of the three levels in LDS), switched on from level 1,
+    with and without the single-workgroup tail, full depth and J=4 (where the
+    fused launch does the last levels): bit-exact to the oracle, forward and
+    inverse, both wavelets."""
+    from oracle.oracle import oracle
+    o = oracle()
+    lib = libdwt_amd.lib()
+    dt = np.float32 if prec == "s" else np.float64
+    rng = np.random.default_rng(9)
+    try:
+        assert lib.dwt_gpu_mid_set(1 << 40) == 0
+        for tb in (32 * 1024, 0):
+            assert lib.dwt_gpu_tail_set(tb) == 0
+            for (W, H) in MID_SHAPES:
+                img = rng.standard_normal((H, W)).astype(dt)
+                for w in cases.WAVELETS:
+                    for J in (-1, 4):
+                        say(f"mid {W}x{H} {w} {prec} J={J} tail {tb}")
+                        ref, jr = o.fwd2(img, w, J)
+                        t = torch.from_numpy(img).cuda()
+                        j = libdwt_amd.fwd2(t, w, J)
+                        assert j == jr
+                        np.testing.assert_array_equal(t.cpu().numpy(), ref, err_msg="forward")
+                        libdwt_amd.inv2(t, w, j)
+                        np.testing.assert_array_equal(t.cpu().numpy(), o.inv2(ref, w, j),
+                                                      err_msg="inverse")
+    finally:
+        lib.dwt_gpu_mid_set(0)
+        lib.dwt_gpu_tail_set(32 * 1024)
+
+
 PYR_SHAPES = [(2, 2), (3, 5), (17, 33), (130, 70), (300, 257), (1000, 40), (9, 250), (513, 511),
               (2048, 3)]
 
