@@ -251,11 +251,55 @@ int dwt_gpu_2d_async(int wavelet, int direction, int precision, void *ptr, int s
                      int size_i_big_y, int *j, int decompose_one, int zero_padding,
                      void *stream);
 
+/* Accuracy statistics of a device-side compare (dwt_gpu_compare). */
+#define DWT_GPU_ULP_BINS 32
+typedef struct dwt_gpu_compare_stats {
+	long long count;      /* pixel pairs compared */
+	long long nonfinite;  /* pairs with a NaN or an infinity on either side */
+	long long over_eps;   /* finite pairs with |a - b| > eps */
+	double max_abs;       /* largest |a - b| over the finite pairs */
+	double sum_sq;        /* sum of (a - b)^2 over the finite pairs */
+	double mse;           /* sum_sq / (count - nonfinite); 0 when none is finite */
+	double peak_b;        /* largest |b| over the finite pairs */
+	double psnr;          /* 10 log10(peak^2 / mse) dB (peak: the argument, else
+	                         peak_b); +inf when mse == 0 */
+	long long ulp[DWT_GPU_ULP_BINS]; /* distance in ulps of a's precision
+	                         between a and b rounded to it: bin 0 equal, bin k
+	                         [2^(k-1), 2^k), the last bin everything above */
+	int differ;           /* dwt_util_compare_s / _d's verdict: 1 when a pair is
+	                         non-finite or further apart than eps */
+	int reserved;
+} dwt_gpu_compare_stats;
+
+/* Compare two images in device memory, element by element, without copying
+ * them to the host: dwt_util_compare_s / _d (src/libdwt.c:1223-1277) with
+ * statistics.  precision_a / _b: DWT_GPU_F32 or DWT_GPU_F64 (may differ);
+ * byte strides as dwt_cdf97_2f_s (element-aligned).  eps < 0 selects the
+ * reference's threshold: 1e-3 when either image is FP32, 1e-6 when both are
+ * FP64; the |a - b| > eps test runs in FP32 when both images are FP32, as the
+ * reference's.  Synchronises `stream`; returns DWT_GPU_OK or an error code. */
+int dwt_gpu_compare(int precision_a, const void *a, int stride_x_a, int stride_y_a,
+                    int precision_b, const void *b, int stride_x_b, int stride_y_b,
+                    int size_x, int size_y, double eps, double peak,
+                    dwt_gpu_compare_stats *stats, void *stream);
+
+/* FP64 verification of an FP32 2D transform on the GPU: `input` (FP32, the
+ * transform's input) is converted to FP64 and transformed in FP64 on the
+ * device (j levels; forward: j as dwt_cdf97_2f_d clamps it), and `result`
+ * (FP32, what the FP32 transform produced from `input`) is compared with it
+ * (eps 1e-3, peak = the largest |coefficient| of the FP64 result).  Both
+ * images are device memory with the same byte strides and a packed frame
+ * (size_o == size_i).  Synchronises `stream`. */
+int dwt_gpu_verify_2d(int wavelet, int direction, const void *input, const void *result,
+                      int stride_x, int stride_y, int size_x, int size_y, int j,
+                      dwt_gpu_compare_stats *stats, void *stream);
+
 /* Per-launch timing with HIP events recorded on the launch stream around each
  * kernel of the packed path.  kind: 0 forward level kernel, 1 inverse level
  * kernel, 2 level-0 detail copy, 4 forward block kernel, 5 inverse block
  * kernel, 6 forward pyramid, 7 inverse pyramid (levels after level 0),
- * 8 / 9 forward / inverse batched level launch;
+ * 8 / 9 forward / inverse batched level launch, 10 / 11 forward / inverse
+ * fused middle levels (dwt_gpu_mid_set);
  * level: the level index, the first level it does for a block kernel or a
  * pyramid (-1 = all).
  * profile_read synchronises on the recorded events. */

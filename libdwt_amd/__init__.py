@@ -33,6 +33,21 @@ class BatchImage(C.Structure):
 
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
 
+ULP_BINS = 32
+
+
+class CompareStats(C.Structure):
+    """struct dwt_gpu_compare_stats."""
+    _fields_ = [("count", C.c_longlong), ("nonfinite", C.c_longlong), ("over_eps", C.c_longlong),
+                ("max_abs", C.c_double), ("sum_sq", C.c_double), ("mse", C.c_double),
+                ("peak_b", C.c_double), ("psnr", C.c_double), ("ulp", C.c_longlong * ULP_BINS),
+                ("differ", C.c_int), ("reserved", C.c_int)]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name not in ("ulp", "reserved")}
+        d["ulp"] = list(self.ulp)
+        return d
+
 
 class LibraryMissing(RuntimeError):
     pass
@@ -126,6 +141,11 @@ def lib() -> C.CDLL:
         L.dwt_gpu_1d_batch.argtypes = [i, i, i, vp, C.c_longlong, C.c_longlong, i, i,
                                        C.POINTER(i), vp]
         L.dwt_gpu_1d_batch.restype = i
+        L.dwt_gpu_compare.argtypes = [i, vp, i, i, i, vp, i, i, i, i, C.c_double, C.c_double,
+                                      C.POINTER(CompareStats), vp]
+        L.dwt_gpu_compare.restype = i
+        L.dwt_gpu_verify_2d.argtypes = [i, i, vp, vp, i, i, i, i, i, C.POINTER(CompareStats), vp]
+        L.dwt_gpu_verify_2d.restype = i
         _lib = L
     return _lib
 
@@ -213,6 +233,42 @@ def lines_1d(wavelet: str, forward: bool, ptr: int, signal_stride: int, sample_s
                                   F64 if f64 else F32, ptr, signal_stride, sample_stride, length,
                                   count, C.byref(lv), stream))
     return lv.value
+
+
+def _image_args(t):
+    """(precision, pointer, stride_x, stride_y, width, height) of a 2D CUDA tensor."""
+    import torch
+    if t.dim() != 2 or t.dtype not in (torch.float32, torch.float64):
+        raise ValueError("expected a 2D float32 or float64 tensor")
+    es = t.element_size()
+    return (F64 if t.dtype == torch.float64 else F32, t.data_ptr(), t.stride(0) * es, t.stride(1) * es,
+            t.shape[1], t.shape[0])
+
+
+def compare(a, b, eps: float = -1.0, peak: float = 0.0, stream: int = 0) -> dict:
+    """dwt_gpu_compare of two same-shape 2D CUDA tensors (any strides, FP32 or
+    FP64 each), on the device; returns the statistics as a dict."""
+    if a.shape != b.shape:
+        raise ValueError("shapes differ")
+    pa, ptr_a, sxa, sya, W, H = _image_args(a)
+    pb, ptr_b, sxb, syb, _, _ = _image_args(b)
+    st = CompareStats()
+    _check(lib().dwt_gpu_compare(pa, ptr_a, sxa, sya, pb, ptr_b, sxb, syb, W, H, eps, peak,
+                                 C.byref(st), stream))
+    return st.as_dict()
+
+
+def verify_2d(inp, result, wavelet: str = "cdf97", forward: bool = True, j: int = -1,
+              stream: int = 0) -> dict:
+    """dwt_gpu_verify_2d: FP32 `result` of the transform of FP32 `inp` against
+    the FP64 transform of `inp` computed on the device."""
+    if inp.shape != result.shape or inp.stride() != result.stride():
+        raise ValueError("input and result must have the same shape and strides")
+    _, ptr_i, sx, sy, W, H = _image_args(inp)
+    st = CompareStats()
+    _check(lib().dwt_gpu_verify_2d(WAVELET_IDS[wavelet], FORWARD if forward else INVERSE, ptr_i,
+                                   result.data_ptr(), sx, sy, W, H, j, C.byref(st), stream))
+    return st.as_dict()
 
 
 def _storage_dtype(storage: str):

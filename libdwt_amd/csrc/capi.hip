@@ -8,6 +8,7 @@
 // fails the call aborts (drop-in) or returns an error (extensions).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -141,7 +142,9 @@ struct PyrPlan {
 struct Scratch {
   // 0: LL of even levels, 1: LL of odd levels, 2: level-0 details,
   // 3: line temp, 4: host staging, 5: LL of the pyramid levels
-  DevBuf buf[6];
+  // 6: compare partials, 7: FP64 reference image of a verification
+  DevBuf buf[8];
+  void* cmp_host = nullptr;  // pinned copy of a compare result
   // pyramid state: plans (most recent first), and the sync words
   // ctrl[0..1], err[2], then per-segment counters
   std::vector<PyrPlan*> plans;
@@ -201,6 +204,8 @@ struct Scratch {
     sync_cnt = 0;
     if (err_host) hipHostFree(err_host);
     err_host = nullptr;
+    if (cmp_host) hipHostFree(cmp_host);
+    cmp_host = nullptr;
     if (trace) hipFree(trace);
     trace = nullptr;
     trace_n = trace_items = 0;
@@ -1199,6 +1204,98 @@ int dwt_gpu_2d_async(int wavelet, int direction, int precision, void* ptr, int s
     else
       run2d<double>(fwd, dptr, stride_x, stride_y, size_o_big_x, size_o_big_y, size_i_big_x,
                     size_i_big_y, j, decompose_one, zero_padding, w, st, false);
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+// ---- device-side compare and FP64 verification ------------------------------------
+static void check_image(const void* p, int sx, int sy, int elem, const char* what) {
+  if (!p) fail(DWT_GPU_EINVAL, "%s: null pointer", what);
+  if (sx % elem || sy % elem || reinterpret_cast<uintptr_t>(p) % elem)
+    fail(DWT_GPU_EINVAL, "%s: pointer and strides must be multiples of the element size", what);
+}
+
+static void compare_device(bool a64, const void* a, int sxa, int sya, bool b64, const void* b, int sxb,
+                           int syb, int W, int H, double eps, double peak, dwt_gpu_compare_stats* out,
+                           hipStream_t st) {
+  std::memset(out, 0, sizeof(*out));
+  out->psnr = INFINITY;
+  if (W == 0 || H == 0) return;
+  const long long n = (long long)W * H;
+  Scratch& sc = scratch_for(st);
+  char* s = (char*)sc.get(6, compare_scratch_bytes(n));
+  if (!sc.cmp_host) HIPCHK(hipHostMalloc(&sc.cmp_host, compare_result_bytes(), hipHostMallocDefault));
+  LAUNCH(launch_compare(a64, a, sxa, sya, b64, b, sxb, syb, W, H, eps, s, st));
+  HIPCHK(hipMemcpyAsync(sc.cmp_host, s + compare_result_offset(n), compare_result_bytes(),
+                        hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  CompareResult r;
+  compare_result(sc.cmp_host, &r);
+  out->count = r.count;
+  out->nonfinite = r.nonfinite;
+  out->over_eps = r.over;
+  out->max_abs = r.max_abs;
+  out->sum_sq = r.sum_sq;
+  out->peak_b = r.peak_b;
+  const long long fin = r.count - r.nonfinite;
+  out->mse = fin > 0 ? r.sum_sq / (double)fin : 0.0;
+  const double pk = peak > 0 ? peak : r.peak_b;
+  out->psnr = out->mse > 0 ? 10.0 * std::log10(pk * pk / out->mse) : INFINITY;
+  for (int k = 0; k < DWT_GPU_ULP_BINS; ++k) out->ulp[k] = r.ulp[k];
+  out->differ = (r.nonfinite > 0 || r.over > 0) ? 1 : 0;
+}
+
+int dwt_gpu_compare(int precision_a, const void* a, int stride_x_a, int stride_y_a, int precision_b,
+                    const void* b, int stride_x_b, int stride_y_b, int size_x, int size_y, double eps,
+                    double peak, dwt_gpu_compare_stats* stats, void* stream) {
+  try {
+    g_err.clear();
+    init_once();
+    for (int pr : {precision_a, precision_b})
+      if (pr != DWT_GPU_F32 && pr != DWT_GPU_F64) fail(DWT_GPU_EINVAL, "bad precision");
+    if (!stats) fail(DWT_GPU_EINVAL, "null stats");
+    if (size_x < 0 || size_y < 0) fail(DWT_GPU_EINVAL, "negative size");
+    const bool a64 = precision_a == DWT_GPU_F64, b64 = precision_b == DWT_GPU_F64;
+    check_image(a, stride_x_a, stride_y_a, a64 ? 8 : 4, "image a");
+    check_image(b, stride_x_b, stride_y_b, b64 ? 8 : 4, "image b");
+    const void* da = device_address(a);
+    const void* db = device_address(b);
+    if (!da || !db) fail(DWT_GPU_EMEMORY, "compare needs device-accessible images");
+    if (eps < 0) eps = (a64 && b64) ? 1e-6 : 1e-3;  // src/libdwt.c:1234, 1261
+    compare_device(a64, da, stride_x_a, stride_y_a, b64, db, stride_x_b, stride_y_b, size_x, size_y,
+                   eps, peak, stats, (hipStream_t)stream);
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_verify_2d(int wavelet, int direction, const void* input, const void* result, int stride_x,
+                      int stride_y, int size_x, int size_y, int j, dwt_gpu_compare_stats* stats,
+                      void* stream) {
+  try {
+    g_err.clear();
+    init_once();
+    if (wavelet != DWT_GPU_CDF97 && wavelet != DWT_GPU_CDF53) fail(DWT_GPU_EINVAL, "bad wavelet");
+    if (direction != DWT_GPU_FORWARD && direction != DWT_GPU_INVERSE) fail(DWT_GPU_EINVAL, "bad direction");
+    if (!stats) fail(DWT_GPU_EINVAL, "null stats");
+    if (size_x < 0 || size_y < 0) fail(DWT_GPU_EINVAL, "negative size");
+    check_image(input, stride_x, stride_y, 4, "input");
+    check_image(result, stride_x, stride_y, 4, "result");
+    const void* din = device_address(input);
+    const void* dres = device_address(result);
+    if (!din || !dres) fail(DWT_GPU_EMEMORY, "verification needs device-accessible images");
+    hipStream_t st = (hipStream_t)stream;
+    Scratch& sc = scratch_for(st);
+    double* ref = (double*)sc.get(7, sizeof(double) * ((size_t)size_x * size_y + 1));
+    LAUNCH(launch_to_f64(din, stride_x, stride_y, ref, size_x, size_y, st));
+    int jj = j;
+    run2d<double>(direction == DWT_GPU_FORWARD, ref, 8 * size_x, 8, size_x, size_y, size_x, size_y, &jj,
+                  0, 0, wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53, st, false);
+    compare_device(false, dres, stride_x, stride_y, true, ref, 8 * size_x, 8, size_x, size_y, 1e-3, 0.0,
+                   stats, st);
     return DWT_GPU_OK;
   } catch (const Failure& f) {
     return f.code;

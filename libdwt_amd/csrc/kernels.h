@@ -64,6 +64,27 @@ void launch_mid_inv(const T* llsrc, long long lp, const T* det, long long dp, T*
                     int W0, int H0, int nl, const Coeffs<T>& cf, hipStream_t st);
 size_t mid_lds_bytes(bool fwd, int nl, int nsteps, int elem);
 hipError_t configure_mid_kernels();
+// Device-side compare of two images (verify.hip): scratch of
+// compare_scratch_bytes(W * H) bytes; the result, compare_result_bytes() at
+// compare_result_offset(W * H), is decoded by compare_result after a copy to
+// the host.  eps: the threshold of the differ verdict.
+constexpr int kUlpBins = 32;
+struct CompareResult {
+  double max_abs, sum_sq, peak_b;
+  long long count, nonfinite, over;
+  long long ulp[kUlpBins];
+};
+long long compare_blocks(long long n);
+size_t compare_scratch_bytes(long long n);
+size_t compare_result_bytes();
+size_t compare_result_offset(long long n);
+void launch_compare(bool a64, const void* a, long long sxa, long long sya, bool b64, const void* b,
+                    long long sxb, long long syb, int W, int H, double eps, void* scratch,
+                    hipStream_t st);
+void compare_result(const void* host_copy, CompareResult* r);
+// FP32 image (byte strides) to a packed FP64 one.
+void launch_to_f64(const void* src, long long sx, long long sy, double* dst, int W, int H,
+                   hipStream_t st);
 // Copy a w x h rectangle between packed buffers.
 template <typename T>
 void launch_copy_rect(const T* src, long long sp, T* dst, long long dp, int w, int h,
