@@ -421,6 +421,15 @@ static int pyr_rows(bool fwd, int W, int H, int nsteps) {
 }
 
 // ---- generic path: per-level line passes in place on any byte strides --------
+// Columns of a layout with contiguous rows (unit column stride, an element
+// pitch) can run as adjacent signals through the coalesced column kernels;
+// lines of one sample keep the line kernel (its N == 1 case).
+template <typename T>
+static bool columns_adjacent(const char* ptr, long long sx, long long sy, int n) {
+  return sy == (long long)sizeof(T) && sx > 0 && sx % (long long)sizeof(T) == 0 &&
+         reinterpret_cast<uintptr_t>(ptr) % sizeof(T) == 0 && n >= 2;
+}
+
 template <typename T>
 static void generic_fwd_level(char* ptr, long long sx, long long sy, int Wo, int Ho, int Wi, int Hi,
                               int j, int zp, const Coeffs<T>& cf, Scratch& sc, hipStream_t st) {
@@ -432,8 +441,22 @@ static void generic_fwd_level(char* ptr, long long sx, long long sy, int Wo, int
   // rows (src/libdwt.c:6354-6362), then columns (6367-6375)
   T* gt = Wis > lim ? (T*)sc.get(3, sizeof(T) * (size_t)Wis * gblocks) : nullptr;
   LAUNCH(launch_line_pass<T>(true, ptr, ptr, ptr + (long long)Wod * sy, sx, sy, Hos, Wis, cf, gt, gblocks, st));
-  gt = His > lim ? (T*)sc.get(3, sizeof(T) * (size_t)His * gblocks) : nullptr;
-  LAUNCH(launch_line_pass<T>(true, ptr, ptr, ptr + (long long)Hod * sx, sy, sx, Wos, His, cf, gt, gblocks, st));
+  if (columns_adjacent<T>(ptr, sx, sy, His)) {
+    // the Wos columns as adjacent signals, one coalesced launch; L and H rows
+    // go to scratch rows [0, ceil(His/2)) and [Hod, Hod + His/2) (the
+    // columns are read while other rows are written), then back into place
+    T* A = reinterpret_cast<T*>(ptr);
+    const long long p = sx / (long long)sizeof(T), Dp = (long long)pad_pitch(Wos);
+    T* D = (T*)sc.get(2, sizeof(T) * (size_t)Dp * Hos);
+    const int half = (His + 1) / 2;
+    LAUNCH(launch_col_level<T>(true, A, p, D, Dp, D + (long long)(Hod - half) * Dp, Dp, His, Wos, cf,
+                               aligned_pair_host<T>(A, p), st));
+    LAUNCH(launch_copy_rect<T>(D, Dp, A, p, Wos, half, st);
+           launch_copy_rect<T>(D + (long long)Hod * Dp, Dp, A + (long long)Hod * p, p, Wos, His / 2, st));
+  } else {
+    gt = His > lim ? (T*)sc.get(3, sizeof(T) * (size_t)His * gblocks) : nullptr;
+    LAUNCH(launch_line_pass<T>(true, ptr, ptr, ptr + (long long)Hod * sx, sy, sx, Wos, His, cf, gt, gblocks, st));
+  }
   if (zp) {  // src/libdwt.c:6377-6397
     const int zx[4] = {(Wis + 1) / 2, Wod, Wod + Wis / 2, Wos};
     const int zy[4] = {(His + 1) / 2, Hod, Hod + His / 2, Hos};
@@ -452,8 +475,20 @@ static void generic_inv_level(char* ptr, long long sx, long long sy, int Wo, int
   // rows (src/libdwt.c:6706-6714), then columns (6719-6727)
   T* gt = Wid > lim ? (T*)sc.get(3, sizeof(T) * (size_t)Wid * gblocks) : nullptr;
   LAUNCH(launch_line_pass<T>(false, ptr, ptr + (long long)Wos * sy, ptr, sx, sy, Hod, Wid, cf, gt, gblocks, st));
-  gt = Hid > lim ? (T*)sc.get(3, sizeof(T) * (size_t)Hid * gblocks) : nullptr;
-  LAUNCH(launch_line_pass<T>(false, ptr, ptr + (long long)Hos * sx, ptr, sy, sx, Wod, Hid, cf, gt, gblocks, st));
+  if (columns_adjacent<T>(ptr, sx, sy, Hid)) {
+    // the Wod columns as adjacent signals: L rows from [0, ceil(Hid/2)), H
+    // rows from [Hos, ..); the Hid output rows to scratch, then into place
+    T* A = reinterpret_cast<T*>(ptr);
+    const long long p = sx / (long long)sizeof(T), Dp = (long long)pad_pitch(Wod);
+    T* D = (T*)sc.get(2, sizeof(T) * (size_t)Dp * Hid);
+    const int half = (Hid + 1) / 2;
+    LAUNCH(launch_col_level<T>(false, A, p, A + (long long)(Hos - half) * p, p, D, Dp, Hid, Wod, cf,
+                               aligned_pair_host<T>(A, p), st));
+    LAUNCH(launch_copy_rect<T>(D, Dp, A, p, Wod, Hid, st));
+  } else {
+    gt = Hid > lim ? (T*)sc.get(3, sizeof(T) * (size_t)Hid * gblocks) : nullptr;
+    LAUNCH(launch_line_pass<T>(false, ptr, ptr + (long long)Hos * sx, ptr, sy, sx, Wod, Hid, cf, gt, gblocks, st));
+  }
   if (zp) {  // src/libdwt.c:6729-6745
     const int zx[4] = {Wid, Wod, 0, 0};
     const int zy[4] = {Hid, Hod, 0, 0};

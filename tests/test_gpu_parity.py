@@ -242,6 +242,40 @@ def test_mid_kernel_shapes(prec):
         lib.dwt_gpu_tail_set(32 * 1024)
 
 
+SPARSE_FRAMES = [((512, 512), (300, 257)), ((1024, 1024), (1000, 700)), ((64, 256), (33, 200)),
+                 ((256, 64), (255, 2))]
+
+
+@pytest.mark.parametrize("prec", ["s", "d"])
+def test_sparse_frames_match_oracle(prec):
+    """Sparse frames (inner image smaller than the outer frame, the rest of the
+    frame random, as the garbage the reference transforms along): rows on the
+    line kernel, columns on the coalesced column kernel through scratch; with
+    and without zero padding and decompose_one, full depth and J=3, forward
+    then inverse, bit-exact to the oracle."""
+    from oracle.oracle import oracle
+    o = oracle()
+    dt = np.float32 if prec == "s" else np.float64
+    es = np.dtype(dt).itemsize
+    rng = np.random.default_rng(17)
+    cases_ = [(f, zp, d1, J) for f in SPARSE_FRAMES for zp in (0, 1) for d1 in (0, 1) for J in (-1, 3)]
+    cases_.append((((4096, 2048), (3001, 2047)), 0, 0, -1))
+    for ((Wo, Ho), (Wi, Hi)), zp, d1, J in cases_:
+        buf = rng.standard_normal((Ho, Wo)).astype(dt)
+        for w in cases.WAVELETS:
+            say(f"sparse {Wo}x{Ho} inner {Wi}x{Hi} {w} {prec} zp={zp} d1={d1} J={J}")
+            ref, jr = o.fwd2(buf, w, J, d1=d1, zp=zp, inner=(Wi, Hi))
+            t = torch.from_numpy(buf).cuda()
+            j = libdwt_amd.transform_2d_async(w, True, t.data_ptr(), Wo * es, es, Wo, Ho, Wi, Hi, j=J,
+                                              decompose_one=d1, zero_padding=zp, f64=prec == "d")
+            assert j == jr
+            np.testing.assert_array_equal(t.cpu().numpy(), ref, err_msg="forward")
+            libdwt_amd.transform_2d_async(w, False, t.data_ptr(), Wo * es, es, Wo, Ho, Wi, Hi, j=j,
+                                          decompose_one=d1, zero_padding=zp, f64=prec == "d")
+            want = o.inv2(ref, w, j, d1=d1, zp=zp, inner=(Wi, Hi))
+            np.testing.assert_array_equal(t.cpu().numpy(), want, err_msg="inverse")
+
+
 PYR_SHAPES = [(2, 2), (3, 5), (17, 33), (130, 70), (300, 257), (1000, 40), (9, 250), (513, 511),
               (2048, 3)]
 
