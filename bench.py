@@ -42,6 +42,41 @@ def measured_traffic(width: int, height: int, wavelet: str):
     return None
 
 
+def bench_sparse(args, torch, libdwt_amd, workloads) -> None:
+    """The width x height image inside a power-of-two outer frame (the
+    reference's DWT_ARR_SPARSE layout), fwd+inv through the drop-in drivers'
+    arguments, against the same image packed and the packed outer frame."""
+    Wi, Hi, J, wv, zp = args.width, args.height, args.levels, args.wavelet, args.zero_padding
+    Wo, Ho = 1 << (Wi - 1).bit_length(), 1 << (Hi - 1).bit_length()
+    buf = torch.zeros((Ho, Wo), device="cuda")
+    buf[:Hi, :Wi] = torch.from_numpy(workloads.sinusoid_image(Wi, Hi, seed=3)).cuda()
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+    res = {}
+    for name, (ox, oy, ix, iy) in (("sparse", (Wo, Ho, Wi, Hi)), ("packed_inner", (Wi, Hi, Wi, Hi)),
+                                   ("packed_outer", (Wo, Ho, Wo, Ho))):
+        pitch = 4 * Wo if name == "sparse" or name == "packed_outer" else 4 * Wi
+
+        def step():
+            j = libdwt_amd.transform_2d_async(wv, True, buf.data_ptr(), pitch, 4, ox, oy, ix, iy, j=J,
+                                              zero_padding=zp, stream=sp)
+            libdwt_amd.transform_2d_async(wv, False, buf.data_ptr(), pitch, 4, ox, oy, ix, iy, j=j,
+                                          zero_padding=zp, stream=sp)
+        for _ in range(args.warmup):
+            step()
+        ms = _timed(torch, stream, step, args.steps)
+        res[name] = {"ms_per_step": round(ms, 5), "Mpix_s": round(Wi * Hi / (ms * 1e-3) / 1e6, 3)}
+    out = {"metric": f"Mpix/s, {Wi}x{Hi} in a {Wo}x{Ho} frame {wv} J={J} fwd+inv",
+           "value": res["sparse"]["Mpix_s"], "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": res["sparse"]["ms_per_step"],
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (sum of 8 random 2D sinusoids + N(0,0.05) noise, seed 3; frame zero)",
+           "config": {"workload": "sparse", "inner": [Wi, Hi], "outer": [Wo, Ho], "levels": J,
+                      "zero_padding": zp, "wavelet": wv},
+           "layouts": res}
+    print(json.dumps(out), flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,7 +93,10 @@ def parse():
                     help="levels after level 0: 1 one pyramid launch, 0 one launch per level")
     ap.add_argument("--cold-steps", type=int, default=10,
                     help="steps timed one at a time after a 512 MiB cache scrub")
-    ap.add_argument("--workload", default="image", choices=("image", "batch", "serve", "lines"),
+    ap.add_argument("--zero-padding", type=int, default=0, choices=(0, 1),
+                    help="sparse: the drivers' zero_padding argument")
+    ap.add_argument("--workload", default="image", choices=("image", "batch", "serve", "lines",
+                                                             "sparse"),
                     help="image: one 4096x4096 image (the headline metric); batch: 256 "
                          "images of uneven sizes, batched launches vs one launch per image; "
                          "serve: Poisson request stream through the serving loop; lines: "
@@ -461,6 +499,9 @@ def main():
         return
     if args.workload == "lines":
         bench_lines(args, torch, libdwt_amd, workloads)
+        return
+    if args.workload == "sparse":
+        bench_sparse(args, torch, libdwt_amd, workloads)
         return
 
     W, H, J, wv = args.width, args.height, args.levels, args.wavelet
