@@ -142,8 +142,9 @@ struct PyrPlan {
 struct Scratch {
   // 0: LL of even levels, 1: LL of odd levels, 2: level-0 details,
   // 3: line temp, 4: host staging, 5: LL of the pyramid levels
-  // 6: compare partials, 7: FP64 reference image of a verification
-  DevBuf buf[8];
+  // 6: compare partials, 7: FP64 reference image of a verification,
+  // 8: packed copy of a restaged frame
+  DevBuf buf[9];
   void* cmp_host = nullptr;  // pinned copy of a compare result
   // pyramid state: plans (most recent first), and the sync words
   // ctrl[0..1], err[2], then per-segment counters
@@ -814,6 +815,43 @@ int level_limit(int Wo, int Ho, int d1) {
   return ceil_log2(d1 ? mx : mn);  // src/libdwt.c:6324
 }
 
+// Frames whose rows are not contiguous (interleaved channels, as the
+// reference's OpenCV wrapper passes them per channel, src/cvdwt.cpp:67-147;
+// transposed images) but whose strides are whole elements are restaged: the
+// Wo x Ho frame is copied into packed scratch, transformed there on the
+// packed paths, and copied back.  The transform depends only on the frame's
+// values and sizes, so the result is the same as in place.
+template <typename T>
+static bool restage_layout(const char* ptr, long long sx, long long sy) {
+  const long long es = sizeof(T);
+  return sy != es && sx != 0 && sy != 0 && sx % es == 0 && sy % es == 0 &&
+         reinterpret_cast<uintptr_t>(ptr) % es == 0;
+}
+
+template <typename T>
+static void fwd2_device(char* ptr, long long sx, long long sy, int Wo, int Ho, int Wi, int Hi,
+                        int* jp, int d1, int zp, Wavelet w, hipStream_t st);
+template <typename T>
+static void inv2_device(char* ptr, long long sx, long long sy, int Wo, int Ho, int Wi, int Hi,
+                        int jmax, int d1, int zp, Wavelet w, hipStream_t st);
+
+template <typename T>
+static T* restage_in(char* ptr, long long sx, long long sy, int Wo, int Ho, long long* ps,
+                     hipStream_t st) {
+  const long long es = sizeof(T);
+  *ps = (long long)pad_pitch(Wo);
+  T* S = (T*)scratch_for(st).get(8, sizeof(T) * (size_t)(*ps) * Ho);
+  LAUNCH(launch_copy_strided<T>(reinterpret_cast<const T*>(ptr), sx / es, sy / es, S, *ps, 1, Wo, Ho, st));
+  return S;
+}
+
+template <typename T>
+static void restage_out(const T* S, long long ps, char* ptr, long long sx, long long sy, int Wo, int Ho,
+                        hipStream_t st) {
+  const long long es = sizeof(T);
+  LAUNCH(launch_copy_strided<T>(S, ps, 1, reinterpret_cast<T*>(ptr), sx / es, sy / es, Wo, Ho, st));
+}
+
 template <typename T>
 static void fwd2_device(char* ptr, long long sx, long long sy, int Wo, int Ho, int Wi, int Hi,
                         int* jp, int d1, int zp, Wavelet w, hipStream_t st) {
@@ -822,6 +860,15 @@ static void fwd2_device(char* ptr, long long sx, long long sy, int Wo, int Ho, i
   if (*jp < 0 || *jp > jl) *jp = jl;  // src/libdwt.c:6326-6329
   const int J = *jp;
   if (J == 0 || Wo <= 0 || Ho <= 0) return;
+  if (restage_layout<T>(ptr, sx, sy)) {
+    long long ps;
+    T* S = restage_in<T>(ptr, sx, sy, Wo, Ho, &ps, st);
+    int j = J;
+    fwd2_device<T>(reinterpret_cast<char*>(S), ps * (long long)sizeof(T), sizeof(T), Wo, Ho, Wi, Hi,
+                   &j, d1, zp, w, st);
+    restage_out<T>(S, ps, ptr, sx, sy, Wo, Ho, st);
+    return;
+  }
   Scratch& sc = scratch_for(st);
   const Coeffs<T> cf = make_coeffs<T>(w, true);
   int jf = 0;
@@ -837,6 +884,14 @@ static void inv2_device(char* ptr, long long sx, long long sy, int Wo, int Ho, i
   int J = level_limit(Wo, Ho, d1);
   if (jmax >= 0 && jmax < J) J = jmax;  // src/libdwt.c:6678-6681
   if (J == 0 || Wo <= 0 || Ho <= 0) return;
+  if (restage_layout<T>(ptr, sx, sy)) {
+    long long ps;
+    T* S = restage_in<T>(ptr, sx, sy, Wo, Ho, &ps, st);
+    inv2_device<T>(reinterpret_cast<char*>(S), ps * (long long)sizeof(T), sizeof(T), Wo, Ho, Wi, Hi, J,
+                   d1, zp, w, st);
+    restage_out<T>(S, ps, ptr, sx, sy, Wo, Ho, st);
+    return;
+  }
   Scratch& sc = scratch_for(st);
   const Coeffs<T> cf = make_coeffs<T>(w, false);
   int jf = 0;

@@ -85,6 +85,11 @@ void compare_result(const void* host_copy, CompareResult* r);
 // FP32 image (byte strides) to a packed FP64 one.
 void launch_to_f64(const void* src, long long sx, long long sy, double* dst, int W, int H,
                    hipStream_t st);
+// Copy a W x H frame between element-strided layouts (row stride r, column
+// stride c, in elements).
+template <typename T>
+void launch_copy_strided(const T* src, long long sr, long long sc, T* dst, long long dr,
+                         long long dc, int W, int H, hipStream_t st);
 // Copy a w x h rectangle between packed buffers.
 template <typename T>
 void launch_copy_rect(const T* src, long long sp, T* dst, long long dp, int w, int h,

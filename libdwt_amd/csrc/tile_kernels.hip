@@ -153,6 +153,29 @@ __global__ void copy_rect_kernel(const T* __restrict__ src, long long sp, T* __r
   }
 }
 
+// Copy of a W x H frame between two element-strided layouts (row stride r,
+// column stride c, in elements, either sign).  The index order follows the
+// source's shorter stride when it gathers (xfast: x varies fastest), so one
+// side of the copy is read or written in order.
+template <typename T>
+__global__ void copy_strided_kernel(const T* __restrict__ src, long long sr, long long sc,
+                                    T* __restrict__ dst, long long dr, long long dc, int W, int H,
+                                    bool xfast) {
+  const long long n = (long long)W * H;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    long long x, y;
+    if (xfast) {
+      y = t / W;
+      x = t - y * W;
+    } else {
+      x = t / H;
+      y = t - x * H;
+    }
+    st_global(dst + y * dr + x * dc, src[y * sr + x * sc]);
+  }
+}
+
 // ---- host launchers ---------------------------------------------------------
 
 // Tile shape of one launch: S rows per segment (tall segments amortise the
@@ -382,6 +405,24 @@ void launch_copy_rect(const T* src, long long sp, T* dst, long long dp, int w, i
                      dst, dp, w, h);
 }
 
+template <typename T>
+void launch_copy_strided(const T* src, long long sr, long long sc, T* dst, long long dr,
+                         long long dc, int W, int H, hipStream_t st) {
+  const long long n = (long long)W * H;
+  if (n <= 0) return;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  // order by the strided side's shorter stride
+  const long long r = (sc == 1 && sr > 0) ? dr : sr, c = (sc == 1 && sr > 0) ? dc : sc;
+  const bool xfast = (c < 0 ? -c : c) <= (r < 0 ? -r : r);
+  hipLaunchKernelGGL((copy_strided_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, src, sr, sc,
+                     dst, dr, dc, W, H, xfast);
+}
+
+template void launch_copy_strided<float>(const float*, long long, long long, float*, long long,
+                                         long long, int, int, hipStream_t);
+template void launch_copy_strided<double>(const double*, long long, long long, double*, long long,
+                                          long long, int, int, hipStream_t);
 template void launch_fwd_level<float>(const float*, long long, int, int, float*, long long,
                                       float*, long long, const Coeffs<float>&, hipStream_t);
 template void launch_fwd_level<double>(const double*, long long, int, int, double*, long long,

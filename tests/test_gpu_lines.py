@@ -43,7 +43,7 @@ def test_lines_match_oracle(layout, prec):
     o = oracle()
     dt = np.float32 if prec == "s" else np.float64
     es = np.dtype(dt).itemsize
-    for N in (2, 3, 5, 777, 1000, 8192):
+    for N in (2, 3, 5, 513, 777, 1000, 4097, 8192, 16384):
         count = 37
         sig = workloads.random_walk(count, N, seed=N).astype(dt)
         for w in ("cdf97", "cdf53"):
@@ -78,6 +78,48 @@ def test_lines_match_oracle(layout, prec):
             libdwt_amd.lines_1d(w, False, args[0], args[1], args[2], N, count, j, prec == "d")
             torch.cuda.synchronize()
             np.testing.assert_array_equal(read(), want_i, err_msg="inverse")
+
+
+@pytest.mark.parametrize("prec", ["s", "d"])
+def test_contiguous_partial_depth(prec):
+    """Contiguous signals, 1..4 levels (the last level's L to global memory),
+    against the oracle's caller loop stopped at that depth."""
+    from oracle.oracle import oracle
+    o = oracle()
+    dt = np.float32 if prec == "s" else np.float64
+    es = np.dtype(dt).itemsize
+    for N in (6, 300, 4096, 5000):
+        sig = workloads.random_walk(9, N, seed=N + 1).astype(dt)
+        for w in ("cdf97", "cdf53"):
+            for lv in (1, 2, 4):
+                lv = min(lv, workloads.ceil_log2(N))  # the forward clamps to ceil(log2 N)
+                say(f"lines rows {prec} N={N} {w} levels={lv}")
+                want = sig.copy()
+                for r in want:  # the caller loop, lv levels
+                    n = N
+                    for _ in range(lv):
+                        lo, hi = o.f_line(r[:n].copy(), w)
+                        r[:len(lo)] = lo
+                        r[len(lo):n] = hi
+                        n = len(lo)
+                t = torch.from_numpy(sig.copy()).cuda()
+                torch.cuda.synchronize()
+                j = libdwt_amd.lines_1d(w, True, t.data_ptr(), N * es, es, N, len(sig), lv,
+                                        prec == "d")
+                torch.cuda.synchronize()
+                assert j == lv
+                np.testing.assert_array_equal(t.cpu().numpy(), want, err_msg="forward")
+                libdwt_amd.lines_1d(w, False, t.data_ptr(), N * es, es, N, len(sig), lv, prec == "d")
+                torch.cuda.synchronize()
+                back = want.copy()
+                sizes = [N]
+                for _ in range(lv - 1):
+                    sizes.append((sizes[-1] + 1) // 2)
+                for r in back:  # the caller loop of inverse levels, coarsest first
+                    for n in reversed(sizes):
+                        h = (n + 1) // 2
+                        r[:n] = o.i_line(r[:h].copy(), r[h:n].copy(), w)
+                np.testing.assert_array_equal(t.cpu().numpy(), back, err_msg="inverse")
 
 
 def test_slice_digests():
