@@ -302,6 +302,10 @@ int dwt_gpu_verify_2d(int wavelet, int direction, const void *input, const void 
  * fused middle levels (dwt_gpu_mid_set);
  * level: the level index, the first level it does for a block kernel or a
  * pyramid (-1 = all).
+ * on: 0 off, 1 on, k >= 2 on with every level-0 forward kernel of the packed
+ * path launched k times back to back inside one event pair (it reads the
+ * image and writes only scratch, so results are unchanged); profile_read
+ * counts launches, so its average is then the back-to-back kernel duration.
  * profile_read synchronises on the recorded events. */
 int dwt_gpu_profile_enable(int on);
 int dwt_gpu_profile_read(int kind, int level, long long *launches, double *total_ms);

@@ -335,8 +335,10 @@ def _max_levels(t) -> int:
     return r
 
 
-def profile_enable(on: bool) -> None:
-    lib().dwt_gpu_profile_enable(1 if on else 0)
+def profile_enable(on: bool, repeat_level0: int = 1) -> None:
+    """Per-launch HIP-event timing; repeat_level0 > 1 runs every level-0
+    forward kernel that many times back to back inside one event pair."""
+    lib().dwt_gpu_profile_enable((repeat_level0 if repeat_level0 > 1 else 1) if on else 0)
 
 
 def profile_reset() -> None:

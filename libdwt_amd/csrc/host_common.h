@@ -59,13 +59,17 @@ int level_limit(int Wo, int Ho, int d1);
 struct ProfRec {
   int kind, level;
   hipEvent_t a, b;
+  int reps;  // launches between the two events
 };
 struct ProfScope {
   bool on = false;
   ProfRec r{};
   hipStream_t st;
-  ProfScope(int kind, int level, hipStream_t s);
+  ProfScope(int kind, int level, hipStream_t s, int reps = 1);
   ~ProfScope();
 };
+// Launches of the level-0 forward kernel per profiled scope (1 unless
+// profiling was enabled with a repeat count).
+int prof_repeat();
 
 }  // namespace dwt
