@@ -44,7 +44,12 @@ struct BatchImage {
   float* scale;
   int W, H, levels, block0;
 };
-constexpr int kMaxAbsPerThread = 16;  // pixels per thread of the max |x| pass
+// The max |x| pass: a block takes maxabs_rows(W) whole rows of one image
+// (about 16 pixels per thread), so no index needs a division.
+constexpr int kMaxAbsPixels = 256 * 16;
+__host__ __device__ inline int maxabs_rows(int W) {
+  return W >= kMaxAbsPixels ? 1 : kMaxAbsPixels / (W > 0 ? W : 1);
+}
 
 void launch_batch_level(bool fwd, int storage, int S, int nblocks, const BatchItem* items,
                         const int* block_item, const Coeffs<float>& cf, hipStream_t st);

@@ -170,43 +170,66 @@ batch_inv_kernel(const BatchItem* __restrict__ items, const int* __restrict__ bl
   }
 }
 
-// Max |x| of every image (atomic max on the bit pattern, which orders
-// non-negative floats); amax must be zero before.
+// Max |x| of every block's rows, as the bit pattern (which orders
+// non-negative floats): one partial per block, reduced per image by the
+// scales kernel.  (One atomic max per block on a word per image serialised
+// ~100 device-scope atomics per image: 200 us for 256 images.)
 __global__ void __launch_bounds__(256)
 batch_maxabs_kernel(const BatchImage* __restrict__ imgs, const int* __restrict__ block_img,
                     unsigned* __restrict__ amax) {
   __shared__ unsigned red[4];
   const int i = block_img[blockIdx.x];
   const BatchImage& im = imgs[i];
-  const long long n = (long long)im.W * im.H;
-  const long long first = (long long)(blockIdx.x - im.block0) * 256 * kMaxAbsPerThread;
-  float m = 0.0f;
-  const Gbl<float>* src = gload_ptr(static_cast<const float*>(im.src));
+  const int W = im.W, R = maxabs_rows(W);
+  const int y0 = (blockIdx.x - im.block0) * R;
+  const int n = min(R, im.H - y0) * W;  // pixels of this block's rows
+  const Gbl<float>* src = gload_ptr(static_cast<const float*>(im.src)) + (long long)y0 * im.sp;
+  // (y, x) of pixel e advance with e by one subtraction loop instead of a
+  // division; all of a thread's loads are issued before any is reduced (one
+  // memory round trip per block)
+  constexpr int K = kMaxAbsPixels / 256;
+  int e = threadIdx.x, y = e / W, x = e - y * W;
+  long long off[K];
 #pragma unroll
-  for (int u = 0; u < kMaxAbsPerThread; ++u) {
-    const long long t = first + (long long)u * 256 + threadIdx.x;
-    if (t < n) {
-      const long long y = t / im.W, x = t - y * im.W;
-      m = fmaxf(m, fabsf(src[y * im.sp + x]));
+  for (int u = 0; u < K; ++u) {
+    off[u] = e < n ? (long long)y * im.sp + x : -1;
+    e += 256;
+    x += 256;
+    while (x >= W) {
+      x -= W;
+      ++y;
     }
   }
+  float v[K];
+#pragma unroll
+  for (int u = 0; u < K; ++u) v[u] = off[u] >= 0 ? fabsf(src[off[u]]) : 0.0f;
+  float m = 0.0f;
+#pragma unroll
+  for (int u = 0; u < K; ++u) m = fmaxf(m, v[u]);
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = __float_as_uint(m);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned v = max(max(red[0], red[1]), max(red[2], red[3]));
-    atomicMax(amax + i, v);
-  }
+  if (threadIdx.x == 0) amax[blockIdx.x] = max(max(red[0], red[1]), max(red[2], red[3]));
 }
 
 // Per-subband scales from max |x|: s = 2^floor(log2(2^14 / (gain * max|x|))).
-__global__ void batch_scales_kernel(const BatchImage* __restrict__ imgs, int n,
-                                    const unsigned* __restrict__ amax, const float* __restrict__ gain) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// One block per image: its threads reduce the image's per-block partials of
+// batch_maxabs_kernel, then one thread per subband computes its scale.
+__global__ void __launch_bounds__(256)
+batch_scales_kernel(const BatchImage* __restrict__ imgs, int n, const unsigned* __restrict__ amax,
+                    const float* __restrict__ gain) {
+  __shared__ unsigned red[4];
+  const int i = blockIdx.x;
   if (i >= n) return;
   const BatchImage& im = imgs[i];
-  const float mx = __uint_as_float(amax[i]);
-  for (int k = 0; k <= 3 * im.levels; ++k) {
+  const int nb = (im.H + maxabs_rows(im.W) - 1) / maxabs_rows(im.W);
+  unsigned mb = 0;
+  for (int b = threadIdx.x; b < nb; b += 256) mb = max(mb, amax[im.block0 + b]);
+  for (int off = 32; off > 0; off >>= 1) mb = max(mb, (unsigned)__shfl_xor((int)mb, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mb;
+  __syncthreads();
+  const float mx = __uint_as_float(max(max(red[0], red[1]), max(red[2], red[3])));
+  for (int k = threadIdx.x; k <= 3 * im.levels; k += 256) {
     // gain table: 3 per level then 32 LL entries (LL after j levels at 96 + j)
     const float gk = k < 3 * im.levels ? gain[k] : gain[96 + im.levels];
     const float bound = gk * mx;
@@ -259,7 +282,7 @@ void launch_batch_maxabs(const BatchImage* imgs, const int* block_img, int nbloc
 void launch_batch_scales(const BatchImage* imgs, int n, const unsigned* amax, const float* gain,
                          hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(batch_scales_kernel, dim3((n + 63) / 64), dim3(64), 0, st, imgs, n, amax, gain);
+  hipLaunchKernelGGL(batch_scales_kernel, dim3(n), dim3(256), 0, st, imgs, n, amax, gain);
 }
 
 // l1 norms of the one-level analysis lowpass and highpass filters (libdwt's

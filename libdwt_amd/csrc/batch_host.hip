@@ -165,8 +165,8 @@ void BatchTables::build(int wavelet_, bool fwd_, int storage_, const dwt_gpu_ima
     for (int i = 0; i < n; ++i) {
       const dwt_gpu_image& im = images[i];
       BatchImage b{im.src, im.src_pitch / 4, im.scales, im.width, im.height, im.levels, (int)imap.size()};
-      const long long px = (long long)im.width * im.height;
-      const long long nb = (px + 256LL * kMaxAbsPerThread - 1) / (256LL * kMaxAbsPerThread);
+      const int rows = maxabs_rows(im.width);
+      const long long nb = (im.height + rows - 1) / rows;
       imap.insert(imap.end(), (size_t)nb, i);
       imgs.push_back(b);
     }
@@ -187,7 +187,7 @@ void BatchTables::build(int wavelet_, bool fwd_, int storage_, const dwt_gpu_ima
   off_imap = off_imgs + al(sizeof(BatchImage) * imgs.size());
   off_gain = off_imap + al(sizeof(int) * imap.size());
   off_amax = off_gain + al(sizeof(float) * gain.size());
-  bytes = off_amax + al(sizeof(unsigned) * (size_t)n);
+  bytes = off_amax + al(sizeof(unsigned) * (imap.empty() ? (size_t)1 : imap.size()));
 }
 
 void BatchTables::pack(char* hp) const {
@@ -203,8 +203,7 @@ void BatchTables::launch(char* dev, hipStream_t st) const {
   if (fwd && storage != kStoreF32) {
     unsigned* amax = reinterpret_cast<unsigned*>(dev + off_amax);
     const BatchImage* im = reinterpret_cast<const BatchImage*>(dev + off_imgs);
-    LAUNCH(launch_fill_u32(amax, 0u, n, st);
-           launch_batch_maxabs(im, reinterpret_cast<const int*>(dev + off_imap), (int)imap.size(), amax, st);
+    LAUNCH(launch_batch_maxabs(im, reinterpret_cast<const int*>(dev + off_imap), (int)imap.size(), amax, st);
            launch_batch_scales(im, n, amax, reinterpret_cast<const float*>(dev + off_gain), st));
   }
   for (const BatchLevel& lv : levels) {
