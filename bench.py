@@ -142,6 +142,8 @@ def parse():
     ap.add_argument("--rate", type=float, default=8000.0,
                     help="serve: offered request rate (requests/s)")
     ap.add_argument("--requests", type=int, default=4000, help="serve: requests per run")
+    ap.add_argument("--client", default="native", choices=("native", "python"),
+                    help="serve: submit from the library's native replay loop or from Python")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -387,13 +389,19 @@ def bench_serve(args, torch, libdwt_amd, workloads):
             s.drain()
             s.poll(1 << 16)
             b0, _ = s.stats()
-            t0 = time.perf_counter()
-            due = t0
-            for (k, wv, fwd, j), out, gap in zip(reqs, outs, gaps):
-                due += gap
-                while time.perf_counter() < due:
-                    pass
-                s.submit(wv, fwd, pool[k], out, j)
+            if args.client == "native":
+                rl = [s.request(wv, fwd, pool[k], out, j) for (k, wv, fwd, j), out in zip(reqs, outs)]
+                at = np.cumsum(gaps).tolist()
+                t0 = time.perf_counter()
+                s.replay(rl, at)
+            else:
+                t0 = time.perf_counter()
+                due = t0
+                for (k, wv, fwd, j), out, gap in zip(reqs, outs, gaps):
+                    due += gap
+                    while time.perf_counter() < due:
+                        pass
+                    s.submit(wv, fwd, pool[k], out, j)
             s.drain()
             t1 = time.perf_counter()
             lat = np.array([x[1] for x in s.poll(1 << 20)])
@@ -415,7 +423,7 @@ def bench_serve(args, torch, libdwt_amd, workloads):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (64 resident sinusoid+noise images, seeds 2000..2063; request mix seed 7)",
         "config": {"workload": "serve", "rate_per_s": args.rate, "requests": n,
-                   "max_batch": 256, "streams": 4, "budget_gib": 16},
+                   "max_batch": 256, "streams": 4, "budget_gib": 16, "client": args.client},
         "batched": batched, "one_request_at_a_time": plain,
         "p99_ratio_plain_over_batched": round(plain["p99_ms"] / batched["p99_ms"], 2),
     }

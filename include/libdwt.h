@@ -438,6 +438,12 @@ typedef struct dwt_gpu_request {
 } dwt_gpu_request;
 int dwt_gpu_server_create(long long hbm_budget_bytes, int max_batch, int streams, void **server);
 int dwt_gpu_server_submit(void *server, const dwt_gpu_request *request, long long *id);
+/* Native load generator: submit reqs[i] at at_s[i] seconds (ascending) after
+ * the call starts, from the calling thread (sleeping through long gaps,
+ * spinning through the last 50 us); ids[i] receives each request's id.
+ * Returns n once all are submitted, or an error code. */
+int dwt_gpu_server_replay(void *server, const dwt_gpu_request *reqs, const double *at_s, int n,
+                          long long *ids);
 /* Up to max completed requests; returns their count. */
 int dwt_gpu_server_poll(void *server, long long *ids, double *latency_s, int *status, int max);
 /* Wait until every submitted request has completed. */

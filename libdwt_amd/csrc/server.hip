@@ -327,6 +327,31 @@ int dwt_gpu_server_submit(void* server, const dwt_gpu_request* req, long long* i
   }
 }
 
+int dwt_gpu_server_replay(void* server, const dwt_gpu_request* reqs, const double* at_s, int n,
+                          long long* ids) {
+  try {
+    g_err.clear();
+    if (!server || (n > 0 && (!reqs || !at_s))) fail(DWT_GPU_EINVAL, "null server, requests or times");
+    Server* sv = static_cast<Server*>(server);
+    using clk = std::chrono::steady_clock;
+    const clk::time_point t0 = clk::now();
+    for (int i = 0; i < n; ++i) {
+      const clk::time_point due =
+          t0 + std::chrono::duration_cast<clk::duration>(std::chrono::duration<double>(at_s[i]));
+      // sleep through long gaps, spin through the last 50 us
+      const clk::time_point wake = due - std::chrono::microseconds(50);
+      if (clk::now() < wake) std::this_thread::sleep_until(wake);
+      while (clk::now() < due) {
+      }
+      const long long k = sv->submit(reqs[i]);
+      if (ids) ids[i] = k;
+    }
+    return n;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
 int dwt_gpu_server_poll(void* server, long long* ids, double* latency_s, int* status, int max) {
   if (!server || max < 0) return DWT_GPU_EINVAL;
   return static_cast<Server*>(server)->poll(ids, latency_s, status, max);

@@ -27,6 +27,9 @@ def _bind(L):
     L.dwt_gpu_server_create.restype = i
     L.dwt_gpu_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_longlong)]
     L.dwt_gpu_server_submit.restype = i
+    L.dwt_gpu_server_replay.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_double), i,
+                                        C.POINTER(C.c_longlong)]
+    L.dwt_gpu_server_replay.restype = i
     L.dwt_gpu_server_poll.argtypes = [vp, C.POINTER(C.c_longlong), C.POINTER(C.c_double),
                                       C.POINTER(i), i]
     L.dwt_gpu_server_poll.restype = i
@@ -69,6 +72,37 @@ class Server:
         im.scales = scales.data_ptr() if scales is not None else None
         libdwt_amd._check(self.L.dwt_gpu_server_submit(self.h, C.byref(r), C.byref(self._id)))
         return self._id.value
+
+    def request(self, wavelet: str, forward: bool, src, dst, levels: int = -1,
+                storage: str = "f32", scales=None) -> Request:
+        """A dwt_gpu_request for src / dst (as submit takes them)."""
+        r = Request()
+        r.wavelet = libdwt_amd.WAVELET_IDS[wavelet]
+        r.direction = libdwt_amd.FORWARD if forward else libdwt_amd.INVERSE
+        r.storage = libdwt_amd.STORAGE[storage]
+        im = r.image
+        H, W = src.shape
+        im.src = src.data_ptr()
+        im.dst = dst.data_ptr()
+        im.src_pitch = src.stride(0) * src.element_size()
+        im.dst_pitch = dst.stride(0) * dst.element_size()
+        im.width = W
+        im.height = H
+        im.levels = levels
+        im.scales = scales.data_ptr() if scales is not None else None
+        return r
+
+    def replay(self, requests, at_s):
+        """Submit requests[i] at at_s[i] seconds after the call, from a native
+        loop (dwt_gpu_server_replay); returns the request ids."""
+        n = len(requests)
+        arr = (Request * n)(*requests)
+        times = (C.c_double * n)(*at_s)
+        ids = (C.c_longlong * n)()
+        rc = self.L.dwt_gpu_server_replay(self.h, arr, times, n, ids)
+        if rc < 0:
+            libdwt_amd._check(rc)
+        return list(ids)
 
     def poll(self, max_n: int = 4096):
         """[(id, latency_s, status)] of requests completed since the last poll."""
