@@ -273,7 +273,6 @@ void init_once() {
     g_init_status = configure_line_kernels();
     if (g_init_status == hipSuccess) g_init_status = configure_tail_kernels();
     if (g_init_status == hipSuccess) g_init_status = configure_mid_kernels();
-    if (g_init_status == hipSuccess) g_init_status = configure_lines_kernels();
   });
   if (g_init_status != hipSuccess)
     fail(DWT_GPU_EHIP, "raising the kernels' LDS limits: %s", hipGetErrorString(g_init_status));
@@ -1078,6 +1077,30 @@ static void run_zero(void* p, int k0, int k1, int stride) {
 }
 
 // ---- full-depth 1D transforms of many signals ---------------------------------------
+// Level 0 of signals stored as matrix columns, in place, as kColStages
+// launches over segment ranges [s[k], s[k+1]) plus one over the top segments
+// [s[kColStages], nseg) whose H goes through scratch.  With R rows per
+// segment, h0 = ceil(n/2) and NS lifting steps:
+//   forward, launched from the top down: segments [a, b) read input rows
+//   [R a - NS, R b + NS) and write H rows h0 + [R a / 2, R b / 2), which
+//   only segments >= b read (already done) when R b + NS <= h0 + R a / 2;
+//   inverse, launched from the bottom up: segments [a, b) write output rows
+//   [R a, R b) and read H rows h0 + [R a / 2 - NS / 2, ...), which lie above
+//   everything written so far when R b <= h0 + R a / 2 - NS / 2.
+// Each boundary is the largest the inequality allows.  Returns false when
+// the signal is too short for strictly increasing boundaries.
+constexpr int kColStages = 3;
+static bool col_stages(bool fwd, int n, int nsteps, int* s) {
+  const int R = col_segment_rows(), h0 = (n + 1) / 2, nseg = col_segments(n);
+  const int slack = fwd ? nsteps : nsteps / 2;
+  s[0] = 0;
+  for (int k = 1; k <= kColStages; ++k) {
+    s[k] = (h0 + R / 2 * s[k - 1] - slack) / R;
+    if (s[k] <= s[k - 1]) return false;
+  }
+  return s[kColStages] < nseg && R / 2 * s[kColStages] - nsteps / 2 >= 0;
+}
+
 // Signal k, sample i at ptr + k * sig + i * smp (bytes).  Levels as the caller
 // loop over dwt_cdf97_f_ex_stride_s: each level on the current n (L to
 // [0, ceil(n/2)), H to [ceil(n/2), n)) while n >= 2.
@@ -1102,9 +1125,72 @@ static void run_lines(bool fwd, char* ptr, long long sig, long long smp, int N, 
   std::vector<int> n(L + 1);
   n[0] = N;
   for (int j = 1; j <= L; ++j) n[j] = (n[j - 1] + 1) / 2;
-  if (aligned && smp == es && sig % es == 0 && N <= lines_lds_max((int)es)) {
-    // contiguous samples: one workgroup per signal, every level in LDS
-    LAUNCH(launch_lines_lds<T>(fwd, reinterpret_cast<T*>(ptr), sig / es, N, count, L, cf, st));
+  if (aligned && smp == es && sig % es == 0 && sig > 0) {
+    // contiguous samples: one launch per level over all signals (rows), the
+    // tile body's row lifting.  Level 0's H overlaps input that other
+    // workgroups have not read yet: a row that fits one workgroup's LDS is
+    // read whole before anything is written (level 0 in place); a longer one
+    // goes through scratch, copied into place on the side stream.  (Per-level
+    // launches measured faster at every length from 16 to 16384 samples than
+    // one workgroup per signal running every level in LDS: 1.20 vs 1.57 ms at
+    // 16384 x 8192, 6.9 vs 63.8 ms at 16 samples per signal.)
+    T* A = reinterpret_cast<T*>(ptr);
+    const long long p = sig / es;
+    const bool vec = (reinterpret_cast<uintptr_t>(ptr) % (2 * es)) == 0 && p % 2 == 0;
+    const int h0 = n[1];
+    const bool res = row_resident_fits(N, (int)es);
+    const long long Dp = (long long)pad_pitch(L == 1 ? N : N - h0);
+    T* D = res ? nullptr : (T*)sc.get(2, sizeof(T) * (size_t)Dp * count);
+    T* P[2] = {nullptr, nullptr};
+    long long Pp[2] = {(long long)pad_pitch(n[L >= 3 ? 2 : 1]), (long long)pad_pitch(n[1])};
+    if (L >= 2) P[1] = (T*)sc.get(1, sizeof(T) * (size_t)Pp[1] * count);
+    if (L >= 3) P[0] = (T*)sc.get(0, sizeof(T) * (size_t)Pp[0] * count);
+    if (!res) sc.ensure_side();
+    if (fwd) {
+      if (res) {
+        LAUNCH(launch_row_resident<T>(true, A, p, L == 1 ? A : P[1], L == 1 ? p : Pp[1], A + h0, p, N, count,
+                                      cf, st));
+      } else {
+        // L == 1: the whole level-0 output (L | H) goes to D and back
+        LAUNCH(launch_row_level<T>(true, A, p, L == 1 ? D : P[1], L == 1 ? Dp : Pp[1],
+                                   L == 1 ? D + h0 : D, Dp, N, count, cf, vec, st));
+        HIPCHK(hipEventRecord(sc.fork, st));
+        HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
+        LAUNCH(launch_copy_rect<T>(D, Dp, L == 1 ? A : A + h0, p, L == 1 ? N : N - h0, count, sc.side));
+      }
+      for (int j = 1; j < L; ++j) {
+        T* out = j == L - 1 ? A : P[(j + 1) % 2];
+        const long long op = j == L - 1 ? p : Pp[(j + 1) % 2];
+        LAUNCH(launch_row_level<T>(true, P[j % 2], Pp[j % 2], out, op, A + n[j + 1], p, n[j], count, cf,
+                                   true, st));
+      }
+      if (!res) {
+        HIPCHK(hipEventRecord(sc.join, sc.side));
+        HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
+      }
+    } else {
+      if (!res) {
+        // level 0's H (all of level 0's input when L == 1) out of A first
+        HIPCHK(hipEventRecord(sc.fork, st));
+        HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
+        LAUNCH(launch_copy_rect<T>(L == 1 ? A : A + h0, p, D, Dp, L == 1 ? N : N - h0, count, sc.side));
+        HIPCHK(hipEventRecord(sc.join, sc.side));
+      }
+      for (int j = L - 1; j >= 1; --j) {
+        const T* lin = j == L - 1 ? A : P[(j + 1) % 2];
+        const long long lp = j == L - 1 ? p : Pp[(j + 1) % 2];
+        LAUNCH(launch_row_level<T>(false, lin, lp, A + n[j + 1], p, P[j % 2], Pp[j % 2], n[j], count, cf,
+                                   true, st));
+      }
+      if (res) {
+        LAUNCH(launch_row_resident<T>(false, L == 1 ? A : P[1], L == 1 ? p : Pp[1], A + h0, p, A, p, N, count,
+                                      cf, st));
+      } else {
+        HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
+        LAUNCH(launch_row_level<T>(false, L == 1 ? D : P[1], L == 1 ? Dp : Pp[1], L == 1 ? D + h0 : D, Dp, A,
+                                   p, N, count, cf, vec, st));
+      }
+    }
     return;
   }
   if (aligned && sig == es && smp % es == 0 && smp / es >= count) {
@@ -1113,20 +1199,42 @@ static void run_lines(bool fwd, char* ptr, long long sig, long long smp, int N, 
     const long long p = smp / es;
     const bool vec = (reinterpret_cast<uintptr_t>(ptr) % (2 * es)) == 0 && p % 2 == 0;
     const long long cp = (long long)pad_pitch(count);
-    T* D = (T*)sc.get(2, sizeof(T) * (size_t)cp * N);
+    const int h0 = n[1], nh0 = N - h0;
+    const int nseg0 = col_segments(N);
+    int stg[kColStages + 1];
+    const bool staged = L >= 2 && col_stages(fwd, N, cf.nsteps, stg);
+    // staged: the top H rows [hs, nh0) pass through scratch (fwd: H rows from
+    // the top segments' first pair; inv: also the rows the lowest of them reads)
+    const int hs = staged ? (fwd ? stg[kColStages] * col_segment_rows() / 2
+                                 : stg[kColStages] * col_segment_rows() / 2 - cf.nsteps / 2)
+                          : 0;
+    T* D = (T*)sc.get(2, sizeof(T) * (size_t)cp * (staged ? nh0 - hs : N));
     T* P[2] = {nullptr, nullptr};
     if (L >= 2) P[1] = (T*)sc.get(1, sizeof(T) * (size_t)cp * n[1]);
     if (L >= 3) P[0] = (T*)sc.get(0, sizeof(T) * (size_t)cp * n[2]);
-    const int h0 = n[1];
     sc.ensure_side();
     if (fwd) {
-      // level 0 into scratch (its H rows overlap unread input), copied into
-      // A on the side stream; later levels read scratch and write A directly
-      LAUNCH(launch_col_level<T>(true, A, p, L == 1 ? D : P[1], cp, D, cp, N, count, cf, vec, st));
-      HIPCHK(hipEventRecord(sc.fork, st));
-      HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
-      LAUNCH(launch_copy_rect<T>(D + (L == 1 ? 0 : (long long)h0 * cp), cp, A + (L == 1 ? 0 : (long long)h0 * p),
-                                 p, count, L == 1 ? N : N - h0, sc.side));
+      if (staged) {
+        // level 0 in place, top segments first: segments [stg[k], stg[k+1])
+        // write H rows that only segments >= stg[k+1] read (done before);
+        // the top segments' H, over rows they read themselves, to scratch
+        LAUNCH(launch_col_segments<T>(true, A, p, P[1], cp, D, cp, N, count, stg[kColStages],
+                                      nseg0 - stg[kColStages], -hs, cf, vec, st));
+        for (int k = kColStages - 1; k >= 0; --k)
+          LAUNCH(launch_col_segments<T>(true, A, p, P[1], cp, A, p, N, count, stg[k], stg[k + 1] - stg[k], h0,
+                                        cf, vec, st));
+        HIPCHK(hipEventRecord(sc.fork, st));
+        HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
+        LAUNCH(launch_copy_rect<T>(D, cp, A + (long long)(h0 + hs) * p, p, count, nh0 - hs, sc.side));
+      } else {
+        // level 0 into scratch (its H rows overlap unread input), copied into
+        // A on the side stream; later levels read scratch and write A directly
+        LAUNCH(launch_col_level<T>(true, A, p, L == 1 ? D : P[1], cp, D, cp, N, count, cf, vec, st));
+        HIPCHK(hipEventRecord(sc.fork, st));
+        HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
+        LAUNCH(launch_copy_rect<T>(D + (L == 1 ? 0 : (long long)h0 * cp), cp, A + (L == 1 ? 0 : (long long)h0 * p),
+                                   p, count, L == 1 ? N : N - h0, sc.side));
+      }
       for (int j = 1; j < L; ++j) {
         T* out = j == L - 1 ? A : P[(j + 1) % 2];
         const long long op = j == L - 1 ? p : cp;
@@ -1137,8 +1245,11 @@ static void run_lines(bool fwd, char* ptr, long long sig, long long smp, int N, 
     } else {
       HIPCHK(hipEventRecord(sc.fork, st));
       HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
-      LAUNCH(launch_copy_rect<T>(A + (L == 1 ? 0 : (long long)h0 * p), p, D + (L == 1 ? 0 : (long long)h0 * cp),
-                                 cp, count, L == 1 ? N : N - h0, sc.side));
+      if (staged)
+        LAUNCH(launch_copy_rect<T>(A + (long long)(h0 + hs) * p, p, D, cp, count, nh0 - hs, sc.side));
+      else
+        LAUNCH(launch_copy_rect<T>(A + (L == 1 ? 0 : (long long)h0 * p), p, D + (L == 1 ? 0 : (long long)h0 * cp),
+                                   cp, count, L == 1 ? N : N - h0, sc.side));
       HIPCHK(hipEventRecord(sc.join, sc.side));
       for (int j = L - 1; j >= 1; --j) {
         const T* llp = j == L - 1 ? A : P[(j + 1) % 2];
@@ -1146,8 +1257,20 @@ static void run_lines(bool fwd, char* ptr, long long sig, long long smp, int N, 
         // H rows always come from A: pair-wise loads only if A allows them
         LAUNCH(launch_col_level<T>(false, llp, lp, A, p, P[j % 2], cp, n[j], count, cf, vec, st));
       }
-      HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
-      LAUNCH(launch_col_level<T>(false, L == 1 ? D : P[1], cp, D, cp, A, p, N, count, cf, true, st));
+      if (staged) {
+        // level 0 in place, bottom segments first: segments [stg[k], stg[k+1])
+        // overwrite only H rows below those that segments >= stg[k] read;
+        // the top segments read their H from the scratch copy
+        for (int k = 0; k < kColStages; ++k)
+          LAUNCH(launch_col_segments<T>(false, P[1], cp, A, p, A, p, N, count, stg[k], stg[k + 1] - stg[k], h0,
+                                        cf, vec, st));
+        HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
+        LAUNCH(launch_col_segments<T>(false, P[1], cp, D, cp, A, p, N, count, stg[kColStages],
+                                      nseg0 - stg[kColStages], -hs, cf, vec, st));
+      } else {
+        HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
+        LAUNCH(launch_col_level<T>(false, L == 1 ? D : P[1], cp, D, cp, A, p, N, count, cf, true, st));
+      }
     }
     return;
   }

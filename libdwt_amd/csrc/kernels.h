@@ -145,12 +145,23 @@ void launch_pyr(bool fwd, bool vec, int S, const PyrTask* d_tasks, int ntasks, i
 void launch_fill_u32(unsigned* p, unsigned v, long long n, hipStream_t st);
 
 // ---- full-depth 1D transforms of many signals (lines.hip) ---------------------------
-// Contiguous samples: one workgroup per signal (signal k at base + k * sig
-// elements), N <= lines_lds_max(sizeof(T)), `levels` levels in LDS.
-int lines_lds_max(int elem);
+// Signals as the rows of a matrix (count rows, contiguous samples): one level
+// on n samples.  Forward: a = input rows (pitch ap), b = L out (pitch bp),
+// c = H out (pitch cp, H sample k at c[k]).  Inverse: a = L in, b = H in
+// (H sample k at b[k]), c = output.  vec: pair-wise access of the input
+// (forward) or output (inverse) rows allowed.
 template <typename T>
-void launch_lines_lds(bool fwd, T* base, long long sig, int N, int count, int levels,
-                      const Coeffs<T>& cf, hipStream_t st);
+void launch_row_level(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c,
+                      long long cp, int n, int count, const Coeffs<T>& cf, bool vec,
+                      hipStream_t st);
+// The same level on whole rows held in one workgroup's LDS: every row is read
+// before any output is written, so the outputs may overlap the input rows
+// (in place).  Same argument meaning; returns false (nothing launched) when a
+// row of n samples is too long to hold or n < 2.
+bool row_resident_fits(int n, int elem);
+template <typename T>
+bool launch_row_resident(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c,
+                         long long cp, int n, int count, const Coeffs<T>& cf, hipStream_t st);
 // Signals as the columns of a matrix (count columns): one level on n rows.
 // Forward: a = input rows (pitch ap), b = L rows out (pitch bp), c = H rows
 // out from row ceil(n/2) (pitch cp).  Inverse: a = L rows, b = H rows from
@@ -159,6 +170,18 @@ template <typename T>
 void launch_col_level(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c,
                       long long cp, int n, int count, const Coeffs<T>& cf, bool vec,
                       hipStream_t st);
-hipError_t configure_lines_kernels();
+// The same level restricted to segments [seg0, seg0 + nseg) of
+// col_segment_rows() rows (col_segments(n) in all), with H row m at row
+// hbase + m of the H buffer (launch_col_level: hbase = ceil(n/2)).  Forward
+// segment s reads input rows [R s - NS, R (s + 1) + NS) and writes output
+// pairs [R s / 2, R (s + 1) / 2); inverse segment s reads H rows
+// [R s / 2 - NS / 2, R (s + 1) / 2 + NS / 2) and writes output rows
+// [R s, R (s + 1)) (R = col_segment_rows(), NS = lifting steps).
+int col_segments(int n);
+int col_segment_rows();
+template <typename T>
+void launch_col_segments(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c,
+                         long long cp, int n, int count, int seg0, int nseg, int hbase,
+                         const Coeffs<T>& cf, bool vec, hipStream_t st);
 
 }  // namespace dwt

@@ -3,14 +3,14 @@
 // level, each level writing L to [0, ceil(n/2)) and H to [ceil(n/2), n) of
 // the signal, until n < 2 (the inverse undoes it level by level).
 //
-// Two layouts have their own kernels:
-//   * contiguous samples (each signal a row): one workgroup per signal holds
-//     the whole signal in LDS and runs every level there, so a signal is read
-//     and written once;
+// Two layouts have their own kernels, one launch per level over all signals:
+//   * contiguous samples (each signal a row): the 2D tile body's row pass; a
+//     wavefront owns a 128-sample window of four rows, lifting in registers
+//     with the neighbour sample by DPP;
 //   * adjacent signals (the signals are the columns of a matrix, sample
-//     stride = row pitch): per level, one launch of a column-only version of
-//     the 2D tile body's vertical sliding window; a wavefront owns 128
-//     adjacent signals (two per lane, pair-wise loads) and a segment of rows.
+//     stride = row pitch): a column-only version of the 2D tile body's
+//     vertical sliding window; a wavefront owns 128 adjacent signals (two per
+//     lane, pair-wise loads) and a segment of rows.
 // The arithmetic is the reference's line lifting (src/libdwt.c:5512-5603,
 // 5779-5870): updates x + c * (l + r) as separate operations, (2c) * n at a
 // line end, scaling after (forward) or before (inverse) the lifting.
@@ -19,146 +19,222 @@
 
 namespace dwt {
 
-constexpr int kLineThreads = 256;
-constexpr int kLineUnroll = 4;
+// ---- row kernels (contiguous signals, one level per launch) ---------------------------
+// A signal is a row of a count x n matrix.  A wavefront owns a 128-sample
+// window of kRowsPerWave rows: lane l holds samples (2l, 2l+1) of the window
+// and the lifting runs in registers with the neighbour sample from the
+// adjacent lane by DPP (the 2D tile body's row pass, row_fwd / row_inv); the
+// core 128 - 2 NS samples of the window are stored.  Forward level on n
+// samples: L to ll (pitch lp) [0, ceil(n/2)), H to hh (pitch hp) [0, n/2).
+constexpr int kRowsPerWave = 4;
 
-// Lifting steps over x[0, n) (n >= 2) by the whole workgroup.  A step's
-// updates are independent (their neighbours have the other parity), so each
-// thread loads the operands of kLineUnroll positions before storing any.  A
-// line end reads its one neighbour twice: c * (n + n) == (2c) * n, the
-// reference's end-of-line form.
-template <typename T>
-__device__ __forceinline__ void lds_lift_line(Lds<T>* x, int n, const Coeffs<T>& cf) {
-  constexpr int STEP = 2 * kLineThreads;
-  for (int st = 0; st < cf.nsteps; ++st) {
-    const int par = (cf.first_parity + st) & 1;
-    const T c = cf.c[st];
-    for (int i0 = par + 2 * (int)threadIdx.x; i0 < n; i0 += kLineUnroll * STEP) {
-      T v[kLineUnroll], l[kLineUnroll], r[kLineUnroll];
+template <typename T, int NS, bool VEC>
+__global__ void __launch_bounds__(256)
+row_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ ll, long long lp,
+               T* __restrict__ hh, long long hp, int n, int count, int nstrips, Coeffs<T> cf) {
+  constexpr int CORE = 2 * kWave - 2 * NS;
+  const int w = __builtin_amdgcn_readfirstlane((int)blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6));
+  const int strip = w % nstrips, r0 = (w / nstrips) * kRowsPerWave;
+  if (r0 >= count) return;
+  const int lane = threadIdx.x & 63;
+  const int Xe = strip * CORE - NS + 2 * lane;
+  const bool interior = __builtin_amdgcn_readfirstlane(strip * CORE - NS >= 0 && strip * CORE - NS + 2 * kWave <= n);
+  const int xe = interior ? Xe : mirror(Xe, n), xo = interior ? Xe + 1 : mirror(Xe + 1, n);
+  const int half = (n + 1) >> 1, nh = n >> 1, k = Xe >> 1;
+  const bool core = lane >= NS / 2 && lane < kWave - NS / 2;
+  T e[kRowsPerWave], o[kRowsPerWave];
+  const Gbl<T>* s = gload_ptr(src);
 #pragma unroll
-      for (int u = 0; u < kLineUnroll; ++u) {
-        const int i = i0 + u * STEP;
-        if (i < n) {
-          v[u] = x[i];
-          l[u] = x[i > 0 ? i - 1 : i + 1];
-          r[u] = x[i + 1 < n ? i + 1 : i - 1];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kLineUnroll; ++u) {
-        const int i = i0 + u * STEP;
-        if (i < n) {
-          T t = l[u] + r[u];
-          t = c * t;
-          x[i] = v[u] + t;
-        }
-      }
+  for (int u = 0; u < kRowsPerWave; ++u) {
+    const int r = min(r0 + u, count - 1);  // rows past the end repeat the last row (not stored)
+    const Gbl<T>* row = s + (long long)r * sp;
+    if (VEC && interior) {
+      using V2 = typename Vec2<T>::type;
+      const V2 v = *(const Gbl<V2>*)(row + Xe);
+      e[u] = v.x;
+      o[u] = v.y;
+    } else {
+      e[u] = row[xe];
+      o[u] = row[xo];
     }
-    __syncthreads();
   }
-}
-
-constexpr int kLineBatch = 8;  // global loads in flight per thread
-
-// x[0, n) = g[0, n): every thread issues kLineBatch loads before it stores
-// any, so that the copy is not a chain of dependent memory round trips.
-template <typename T>
-__device__ __forceinline__ void copy_in(Lds<T>* x, const Gbl<T>* g, int n) {
-  for (int i0 = threadIdx.x; i0 < n; i0 += kLineBatch * kLineThreads) {
-    T v[kLineBatch];
 #pragma unroll
-    for (int u = 0; u < kLineBatch; ++u) {
-      const int i = i0 + u * kLineThreads;
-      if (i < n) v[u] = g[i];
-    }
-#pragma unroll
-    for (int u = 0; u < kLineBatch; ++u) {
-      const int i = i0 + u * kLineThreads;
-      if (i < n) x[i] = v[u];
+  for (int u = 0; u < kRowsPerWave; ++u) {
+    row_fwd<T, NS>(e[u], o[u], cf);
+    const int r = r0 + u;
+    if (core && r < count) {
+      if (k < half) st_global(ll + (long long)r * lp + k, e[u]);
+      if (k < nh) st_global(hh + (long long)r * hp + k, o[u]);
     }
   }
 }
 
-// One workgroup per signal: signal k at base + k * sig (elements), contiguous
-// samples; `levels` levels while n >= 2.  LDS: buffers A (N elements) and B
-// (ceil(N/2)); the level whose input (forward) or output (inverse) has n
-// samples uses A when its index is even, B when odd.  Forward: lift in place,
-// then H (odd samples x so) goes straight to its final place in global memory
-// and L (even samples x se) to the other buffer for the next level.  Inverse:
-// L (x se) from the buffer and H (x so) from global memory interleaved into
-// the other buffer, then lifted.  Every sample is read once and written once.
-template <typename T, bool FWD>
-__global__ void __launch_bounds__(kLineThreads)
-lines_lds_kernel(T* __restrict__ base, long long sig, int N, int levels, Coeffs<T> cf) {
-  extern __shared__ __align__(16) unsigned char line_smem[];
-  Lds<T>* const A = (Lds<T>*)line_smem;  // generic -> LDS address-space cast
-  Lds<T>* const B = A + N;
-  Gbl<T>* const g = gstore_ptr(base + (long long)blockIdx.x * sig);
-  const int t = threadIdx.x;
-  if (FWD) {
-    copy_in(A, g, N);
-    __syncthreads();
-    int n = N, l = 0;
-    for (; l < levels && n >= 2; ++l) {
-      Lds<T>* X = (l & 1) ? B : A;
-      Lds<T>* Y = (l & 1) ? A : B;
-      lds_lift_line(X, n, cf);
-      const int half = (n + 1) >> 1;
-      for (int i = t; i < n; i += kLineThreads) {
-        const T v = X[i];
-        if (i & 1) g[half + (i >> 1)] = v * cf.so;
-        else Y[i >> 1] = v * cf.se;
-      }
-      __syncthreads();
-      n = half;
-    }
-    Lds<T>* X = (l & 1) ? B : A;
-    for (int i = t; i < n; i += kLineThreads) g[i] = X[i];
-  } else {
-    int sizes[32];
-    int L = 0;
-    for (int n = N; n >= 2 && L < levels; n = (n + 1) >> 1) sizes[L++] = n;
-    const int n0 = L ? (sizes[L - 1] + 1) >> 1 : N;  // coarsest L
-    Lds<T>* X = (L & 1) ? B : A;
-    copy_in(X, g, n0);
-    __syncthreads();
-    for (int l = L - 1; l >= 0; --l) {
-      const int n = sizes[l], half = (n + 1) >> 1;
-      Lds<T>* Xi = (l & 1) ? A : B;  // input: level l + 1's buffer
-      Lds<T>* Y = (l & 1) ? B : A;
-      // H from global memory, kLineBatch loads in flight per thread
-      for (int k0 = t; k0 < n - half; k0 += kLineBatch * kLineThreads) {
-        T h[kLineBatch];
+// Inverse level on n output samples: L from ll (pitch lp), H from hh (pitch
+// hp); output to dst (pitch dp).
+template <typename T, int NS, bool VEC>
+__global__ void __launch_bounds__(256)
+row_inv_kernel(const T* __restrict__ ll, long long lp, const T* __restrict__ hh, long long hp,
+               T* __restrict__ dst, long long dp, int n, int count, int nstrips, Coeffs<T> cf) {
+  constexpr int CORE = 2 * kWave - 2 * NS;
+  const int w = __builtin_amdgcn_readfirstlane((int)blockIdx.x * kWavesPerBlock + (int)(threadIdx.x >> 6));
+  const int strip = w % nstrips, r0 = (w / nstrips) * kRowsPerWave;
+  if (r0 >= count) return;
+  const int lane = threadIdx.x & 63;
+  const int Xe = strip * CORE - NS + 2 * lane;
+  const int ke = mirror(Xe, n) >> 1, ko = mirror(Xe + 1, n) >> 1;  // parity is kept by mirroring
+  const bool core = lane >= NS / 2 && lane < kWave - NS / 2;
+  T e[kRowsPerWave], o[kRowsPerWave];
+  const Gbl<T>* lg = gload_ptr(ll);
+  const Gbl<T>* hg = gload_ptr(hh);
 #pragma unroll
-        for (int u = 0; u < kLineBatch; ++u) {
-          const int k = k0 + u * kLineThreads;
-          if (k < n - half) h[u] = g[half + k];
-        }
+  for (int u = 0; u < kRowsPerWave; ++u) {
+    const int r = min(r0 + u, count - 1);
+    e[u] = lg[(long long)r * lp + ke];
+    o[u] = hg[(long long)r * hp + ko];
+  }
 #pragma unroll
-        for (int u = 0; u < kLineBatch; ++u) {
-          const int k = k0 + u * kLineThreads;
-          if (k < n - half) Y[2 * k + 1] = h[u] * cf.so;
-        }
+  for (int u = 0; u < kRowsPerWave; ++u) {
+    row_inv<T, NS>(e[u], o[u], cf);
+    const int r = r0 + u;
+    if (core && r < count) {
+      T* out = dst + (long long)r * dp + Xe;
+      if (VEC && Xe + 1 < n) {
+        using V2 = typename Vec2<T>::type;
+        V2 v;
+        v.x = e[u];
+        v.y = o[u];
+        *gstore_ptr(reinterpret_cast<V2*>(out)) = v;
+      } else {
+        if (Xe < n) st_global(out, e[u]);
+        if (Xe + 1 < n) st_global(out + 1, o[u]);
       }
-      for (int k = t; k < half; k += kLineThreads) Y[2 * k] = Xi[k] * cf.se;
-      __syncthreads();
-      lds_lift_line(Y, n, cf);
     }
-    for (int i = t; i < N; i += kLineThreads) g[i] = A[i];
+  }
+}
+
+// ---- resident row kernels (level 0 in place) -----------------------------------------
+// A workgroup owns R whole rows: it reads them into LDS, waits, and only then
+// writes the level's output, so the output may overlap the rows it read (the
+// H half of level 0 goes straight to its final place, no scratch copy).  The
+// windows (R rows x nstrips) are shared among the workgroup's wavefronts and
+// computed exactly as in row_fwd_kernel / row_inv_kernel.  Row u sits at
+// lds[u * ln], ln = n rounded up to a multiple of four.
+constexpr int kResBytes = 32768;     // LDS per workgroup (rows packed up to this)
+constexpr int kResMaxBytes = 65536;  // longest row held whole
+
+template <typename T, int VW> struct VecN { typedef T type __attribute__((ext_vector_type(VW))); };
+template <typename T> struct VecN<T, 1> { typedef T type; };
+
+// Read two segments of each of `rows` rows into LDS: row u's m1 elements at
+// g1 + u * p1 to l + u * ln, its m2 elements at g2 + u * p2 to l + u * ln + m1.
+// VW elements per access (pointers, pitches, m1, m2 and ln multiples of VW).
+template <typename T, int VW>
+__device__ __forceinline__ void res_fill(T* l, int ln, int rows, const T* g1, long long p1, int m1,
+                                         const T* g2, long long p2, int m2) {
+  using V = typename VecN<T, VW>::type;
+  const int v1 = m1 / VW, v = v1 + m2 / VW;
+  for (int idx = threadIdx.x; idx < rows * v; idx += blockDim.x) {
+    const int u = idx / v, i = idx - u * v;
+    const T* g = i < v1 ? g1 + u * p1 + (long long)i * VW : g2 + u * p2 + (long long)(i - v1) * VW;
+    *reinterpret_cast<V*>(l + u * ln + i * VW) = *reinterpret_cast<const Gbl<V>*>(gload_ptr(g));
+  }
+}
+
+template <typename T, int NS, int VW>
+__global__ void __launch_bounds__(256)
+row_res_fwd_kernel(const T* src, long long sp, T* ll, long long lp, T* hh, long long hp, int n,
+                   int count, int R, int nstrips, Coeffs<T> cf) {
+  extern __shared__ __align__(16) unsigned char res_smem[];
+  T* lds = reinterpret_cast<T*>(res_smem);
+  constexpr int CORE = 2 * kWave - 2 * NS;
+  const int ln = (n + 3) & ~3;
+  const int r0 = blockIdx.x * R;
+  const int rows = min(R, count - r0);
+  res_fill<T, VW>(lds, ln, rows, src + (long long)r0 * sp, sp, n, src, 0, 0);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int half = (n + 1) >> 1, nh = n >> 1;
+  const bool core = lane >= NS / 2 && lane < kWave - NS / 2;
+  for (int it = wave; it < rows * nstrips; it += nw) {
+    const int u = it / nstrips, strip = it - u * nstrips;
+    const int Xe = strip * CORE - NS + 2 * lane;
+    const bool interior = strip * CORE - NS >= 0 && strip * CORE - NS + 2 * kWave <= n;
+    const T* row = lds + u * ln;
+    T e, o;
+    if (interior) {
+      using V2 = typename Vec2<T>::type;
+      const V2 v = *reinterpret_cast<const V2*>(row + Xe);
+      e = v.x;
+      o = v.y;
+    } else {
+      e = row[mirror(Xe, n)];
+      o = row[mirror(Xe + 1, n)];
+    }
+    row_fwd<T, NS>(e, o, cf);
+    const int k = Xe >> 1;
+    const long long r = r0 + u;
+    if (core) {
+      if (k < half) st_global(ll + r * lp + k, e);
+      if (k < nh) st_global(hh + r * hp + k, o);
+    }
+  }
+}
+
+// Inverse level on n output samples: L (ceil(n/2)) from ll, H (n/2) from hh;
+// output to dst.  In LDS, H follows L at offset ceil(n/2).
+template <typename T, int NS, int VW, bool VEC>
+__global__ void __launch_bounds__(256)
+row_res_inv_kernel(const T* ll, long long lp, const T* hh, long long hp, T* dst, long long dp, int n,
+                   int count, int R, int nstrips, Coeffs<T> cf) {
+  extern __shared__ __align__(16) unsigned char res_smem[];
+  T* lds = reinterpret_cast<T*>(res_smem);
+  constexpr int CORE = 2 * kWave - 2 * NS;
+  const int ln = (n + 3) & ~3;
+  const int half = (n + 1) >> 1, nh = n >> 1;
+  const int r0 = blockIdx.x * R;
+  const int rows = min(R, count - r0);
+  res_fill<T, VW>(lds, ln, rows, ll + (long long)r0 * lp, lp, half, hh + (long long)r0 * hp, hp, nh);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const bool core = lane >= NS / 2 && lane < kWave - NS / 2;
+  for (int it = wave; it < rows * nstrips; it += nw) {
+    const int u = it / nstrips, strip = it - u * nstrips;
+    const int Xe = strip * CORE - NS + 2 * lane;
+    const int ke = mirror(Xe, n) >> 1, ko = mirror(Xe + 1, n) >> 1;
+    const T* row = lds + u * ln;
+    T e = row[ke], o = row[half + ko];
+    row_inv<T, NS>(e, o, cf);
+    if (core) {
+      T* out = dst + (long long)(r0 + u) * dp + Xe;
+      if (VEC && Xe + 1 < n) {
+        using V2 = typename Vec2<T>::type;
+        V2 v;
+        v.x = e;
+        v.y = o;
+        *gstore_ptr(reinterpret_cast<V2*>(out)) = v;
+      } else {
+        if (Xe < n) st_global(out, e);
+        if (Xe + 1 < n) st_global(out + 1, o);
+      }
+    }
   }
 }
 
 // ---- column kernels (adjacent signals) ---------------------------------------------
 // A signal is a column: sample i of signal x at row i.  Forward level on n
-// rows: src (row pitch sp) -> L rows to ll (pitch llp), H rows to det (pitch
-// dp) starting at row ceil(n/2).  count signals; wave = 128 columns.
+// rows: src (row pitch sp) -> L rows to ll (pitch llp), H row m to det row
+// hbase + m (pitch dp; hbase = ceil(n/2) puts H in place).  count signals;
+// wave = 128 columns.  A launch computes segments [seg0, seg0 + nseg) of SS
+// rows (SS / 2 output pairs each); segment s reads input rows
+// [SS s - NS, SS (s + 1) + NS) (mirrored at the ends).
 template <typename T, int NS, int SS, bool VEC>
 __global__ void __launch_bounds__(256)
 col_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ ll, long long llp,
-               T* __restrict__ det, long long dp, int n, int count, int nseg, Coeffs<T> cf) {
+               T* __restrict__ det, long long dp, int n, int count, int nseg, int seg0, int hbase,
+               Coeffs<T> cf) {
   constexpr int LAG = NS / 2, NP = SS / 2 + NS;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int g = b / nseg, seg = b - g * nseg;
+  const int g = b / nseg, seg = seg0 + (b - g * nseg);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int x = (g * kWavesPerBlock + wave) * 128 + 2 * lane;
@@ -193,22 +269,25 @@ col_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ ll, long
       if (va) st_global(ll + (long long)m * llp + x, La);
       if (vb) st_global(ll + (long long)m * llp + x + 1, Lb);
       if (m < nh) {
-        if (va) st_global(det + (long long)(half + m) * dp + x, Ha);
-        if (vb) st_global(det + (long long)(half + m) * dp + x + 1, Hb);
+        if (va) st_global(det + (long long)(hbase + m) * dp + x, Ha);
+        if (vb) st_global(det + (long long)(hbase + m) * dp + x + 1, Hb);
       }
     }
   }
 }
 
-// Inverse level on n output rows: L rows from ll (pitch llp), H rows from det
-// (pitch dp) starting at row ceil(n/2); output rows to dst (pitch dstp).
+// Inverse level on n output rows: L rows from ll (pitch llp), H row m from
+// det row hbase + m (pitch dp); output rows to dst (pitch dstp).  Segment s
+// writes output rows [SS s, SS (s + 1)) and reads H rows
+// [SS s / 2 - NS / 2, SS (s + 1) / 2 + NS / 2).
 template <typename T, int NS, int SS, bool VEC>
 __global__ void __launch_bounds__(256)
 col_inv_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ det, long long dp,
-               T* __restrict__ dst, long long dstp, int n, int count, int nseg, Coeffs<T> cf) {
+               T* __restrict__ dst, long long dstp, int n, int count, int nseg, int seg0, int hbase,
+               Coeffs<T> cf) {
   constexpr int LAG = NS / 2, NP = SS / 2 + NS;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int g = b / nseg, seg = b - g * nseg;
+  const int g = b / nseg, seg = seg0 + (b - g * nseg);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int x = (g * kWavesPerBlock + wave) * 128 + 2 * lane;
@@ -224,7 +303,7 @@ col_inv_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ de
   for (int u = 0; u < 2 * NP; ++u) {
     const int Yr = mirror_near(2 * pbeg + u, n);
     const int m = Yr >> 1;
-    const Gbl<T>* row = (Yr & 1) ? hp + (long long)(half + m) * dp + x : lp + (long long)m * llp + x;
+    const Gbl<T>* row = (Yr & 1) ? hp + (long long)(hbase + m) * dp + x : lp + (long long)m * llp + x;
     if (VEC && vb) {
       using V2 = typename Vec2<T>::type;
       const V2 v = *(const Gbl<V2>*)row;
@@ -255,69 +334,145 @@ col_inv_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ de
 }
 
 // ---- host launchers ------------------------------------------------------------------
-// Longest signal of the LDS kernel: N + ceil(N/2) elements in 96 KB.
-int lines_lds_max(int elem) { return (2 * 96 * 1024 / 3) / elem; }
-
-template <typename T>
-void launch_lines_lds(bool fwd, T* base, long long sig, int N, int count, int levels,
-                      const Coeffs<T>& cf, hipStream_t st) {
-  const size_t lds = sizeof(T) * ((size_t)N + (N + 1) / 2);
-  if (fwd)
-    hipLaunchKernelGGL((lines_lds_kernel<T, true>), dim3(count), dim3(kLineThreads), lds, st, base,
-                       sig, N, levels, cf);
-  else
-    hipLaunchKernelGGL((lines_lds_kernel<T, false>), dim3(count), dim3(kLineThreads), lds, st,
-                       base, sig, N, levels, cf);
-}
-
 constexpr int kColRows = 8;
 
 template <typename T, int NS, bool VEC>
 static void col_go(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c, long long cp,
-                   int n, int count, const Coeffs<T>& cf, hipStream_t st) {
-  const int nseg = (n + kColRows - 1) / kColRows;
+                   int n, int count, int seg0, int nseg, int hbase, const Coeffs<T>& cf,
+                   hipStream_t st) {
+  if (nseg <= 0) return;
   const int ngroups = (count + 128 * kWavesPerBlock - 1) / (128 * kWavesPerBlock);
   const dim3 grid(ngroups * nseg), block(256);
   if (fwd)  // a = src, b = ll (written), c = det
     hipLaunchKernelGGL((col_fwd_kernel<T, NS, kColRows, VEC>), grid, block, 0, st, a, ap,
-                       const_cast<T*>(b), bp, c, cp, n, count, nseg, cf);
+                       const_cast<T*>(b), bp, c, cp, n, count, nseg, seg0, hbase, cf);
   else  // a = ll, b = det, c = dst
     hipLaunchKernelGGL((col_inv_kernel<T, NS, kColRows, VEC>), grid, block, 0, st, a, ap, b, bp, c,
-                       cp, n, count, nseg, cf);
+                       cp, n, count, nseg, seg0, hbase, cf);
+}
+
+template <typename T, int NS, bool VEC>
+static void row_go(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c, long long cp,
+                   int n, int count, const Coeffs<T>& cf, hipStream_t st) {
+  constexpr int CORE = 2 * kWave - 2 * NS;
+  const int nstrips = (n + CORE - 1) / CORE;
+  const long long waves = (long long)nstrips * ((count + kRowsPerWave - 1) / kRowsPerWave);
+  const dim3 grid((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock)), block(kWave * kWavesPerBlock);
+  if (fwd)  // a = src, b = L out, c = H out
+    hipLaunchKernelGGL((row_fwd_kernel<T, NS, VEC>), grid, block, 0, st, a, ap, const_cast<T*>(b), bp,
+                       c, cp, n, count, nstrips, cf);
+  else  // a = L in, b = H in, c = output
+    hipLaunchKernelGGL((row_inv_kernel<T, NS, VEC>), grid, block, 0, st, a, ap, b, bp, c, cp, n, count,
+                       nstrips, cf);
+}
+
+template <typename T, int NS, int VW>
+static void res_go(bool fwd, bool vec, const T* a, long long ap, const T* b, long long bp, T* c,
+                   long long cp, int n, int count, const Coeffs<T>& cf, hipStream_t st) {
+  constexpr int CORE = 2 * kWave - 2 * NS;
+  const int nstrips = (n + CORE - 1) / CORE;
+  const long long row_bytes = (long long)((n + 3) & ~3) * (long long)sizeof(T);
+  const int R = (int)std::max(1LL, std::min((long long)count, kResBytes / row_bytes));
+  const dim3 grid((unsigned)((count + R - 1) / R)), block(kWave * kWavesPerBlock);
+  const size_t lds = (size_t)(R * row_bytes);
+  if (fwd)  // a = src, b = L out, c = H out
+    hipLaunchKernelGGL((row_res_fwd_kernel<T, NS, VW>), grid, block, lds, st, a, ap, const_cast<T*>(b),
+                       bp, c, cp, n, count, R, nstrips, cf);
+  else if (vec)  // a = L in, b = H in, c = output
+    hipLaunchKernelGGL((row_res_inv_kernel<T, NS, VW, true>), grid, block, lds, st, a, ap, b, bp, c, cp,
+                       n, count, R, nstrips, cf);
+  else
+    hipLaunchKernelGGL((row_res_inv_kernel<T, NS, VW, false>), grid, block, lds, st, a, ap, b, bp, c, cp,
+                       n, count, R, nstrips, cf);
+}
+
+template <typename T, int NS>
+static void res_width(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c, long long cp,
+                      int n, int count, const Coeffs<T>& cf, hipStream_t st) {
+  const long long es = sizeof(T);
+  auto fits = [&](int vw) {
+    auto ok = [&](const void* p, long long pitch, long long m) {
+      return reinterpret_cast<uintptr_t>(p) % (uintptr_t)(vw * es) == 0 && pitch % vw == 0 && m % vw == 0;
+    };
+    return fwd ? ok(a, ap, n) : ok(a, ap, (n + 1) / 2) && ok(b, bp, n / 2);
+  };
+  const bool vec = !fwd && reinterpret_cast<uintptr_t>(c) % (uintptr_t)(2 * es) == 0 && cp % 2 == 0;
+  if (es == 4 && fits(4)) res_go<T, NS, 4>(fwd, vec, a, ap, b, bp, c, cp, n, count, cf, st);
+  else if (fits(2)) res_go<T, NS, 2>(fwd, vec, a, ap, b, bp, c, cp, n, count, cf, st);
+  else res_go<T, NS, 1>(fwd, vec, a, ap, b, bp, c, cp, n, count, cf, st);
+}
+
+bool row_resident_fits(int n, int elem) {
+  return n >= 2 && (long long)((n + 3) & ~3) * elem <= kResMaxBytes;
+}
+
+template <typename T>
+bool launch_row_resident(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c,
+                         long long cp, int n, int count, const Coeffs<T>& cf, hipStream_t st) {
+  if (!row_resident_fits(n, (int)sizeof(T))) return false;
+  if (count > 0) {
+    if (cf.nsteps == 4) res_width<T, 4>(fwd, a, ap, b, bp, c, cp, n, count, cf, st);
+    else res_width<T, 2>(fwd, a, ap, b, bp, c, cp, n, count, cf, st);
+  }
+  return true;
+}
+
+template bool launch_row_resident<float>(bool, const float*, long long, const float*, long long, float*,
+                                         long long, int, int, const Coeffs<float>&, hipStream_t);
+template bool launch_row_resident<double>(bool, const double*, long long, const double*, long long,
+                                          double*, long long, int, int, const Coeffs<double>&,
+                                          hipStream_t);
+
+template <typename T>
+void launch_row_level(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c,
+                      long long cp, int n, int count, const Coeffs<T>& cf, bool vec,
+                      hipStream_t st) {
+  if (cf.nsteps == 4) {
+    if (vec) row_go<T, 4, true>(fwd, a, ap, b, bp, c, cp, n, count, cf, st);
+    else row_go<T, 4, false>(fwd, a, ap, b, bp, c, cp, n, count, cf, st);
+  } else {
+    if (vec) row_go<T, 2, true>(fwd, a, ap, b, bp, c, cp, n, count, cf, st);
+    else row_go<T, 2, false>(fwd, a, ap, b, bp, c, cp, n, count, cf, st);
+  }
+}
+
+template void launch_row_level<float>(bool, const float*, long long, const float*, long long, float*,
+                                      long long, int, int, const Coeffs<float>&, bool, hipStream_t);
+template void launch_row_level<double>(bool, const double*, long long, const double*, long long,
+                                       double*, long long, int, int, const Coeffs<double>&, bool,
+                                       hipStream_t);
+
+template <typename T>
+void launch_col_segments(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c,
+                         long long cp, int n, int count, int seg0, int nseg, int hbase,
+                         const Coeffs<T>& cf, bool vec, hipStream_t st) {
+  if (cf.nsteps == 4) {
+    if (vec) col_go<T, 4, true>(fwd, a, ap, b, bp, c, cp, n, count, seg0, nseg, hbase, cf, st);
+    else col_go<T, 4, false>(fwd, a, ap, b, bp, c, cp, n, count, seg0, nseg, hbase, cf, st);
+  } else {
+    if (vec) col_go<T, 2, true>(fwd, a, ap, b, bp, c, cp, n, count, seg0, nseg, hbase, cf, st);
+    else col_go<T, 2, false>(fwd, a, ap, b, bp, c, cp, n, count, seg0, nseg, hbase, cf, st);
+  }
 }
 
 template <typename T>
 void launch_col_level(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c,
                       long long cp, int n, int count, const Coeffs<T>& cf, bool vec,
                       hipStream_t st) {
-  if (cf.nsteps == 4) {
-    if (vec) col_go<T, 4, true>(fwd, a, ap, b, bp, c, cp, n, count, cf, st);
-    else col_go<T, 4, false>(fwd, a, ap, b, bp, c, cp, n, count, cf, st);
-  } else {
-    if (vec) col_go<T, 2, true>(fwd, a, ap, b, bp, c, cp, n, count, cf, st);
-    else col_go<T, 2, false>(fwd, a, ap, b, bp, c, cp, n, count, cf, st);
-  }
+  launch_col_segments<T>(fwd, a, ap, b, bp, c, cp, n, count, 0, col_segments(n), (n + 1) / 2, cf, vec,
+                         st);
 }
 
-hipError_t configure_lines_kernels() {
-  const void* fns[] = {reinterpret_cast<const void*>(&lines_lds_kernel<float, true>),
-                       reinterpret_cast<const void*>(&lines_lds_kernel<float, false>),
-                       reinterpret_cast<const void*>(&lines_lds_kernel<double, true>),
-                       reinterpret_cast<const void*>(&lines_lds_kernel<double, false>)};
-  for (const void* f : fns) {
-    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      return e;
-    }
-  }
-  return hipSuccess;
-}
+int col_segments(int n) { return (n + kColRows - 1) / kColRows; }
+int col_segment_rows() { return kColRows; }
 
-template void launch_lines_lds<float>(bool, float*, long long, int, int, int, const Coeffs<float>&,
-                                      hipStream_t);
-template void launch_lines_lds<double>(bool, double*, long long, int, int, int,
-                                       const Coeffs<double>&, hipStream_t);
+template void launch_col_segments<float>(bool, const float*, long long, const float*, long long, float*,
+                                         long long, int, int, int, int, int, const Coeffs<float>&, bool,
+                                         hipStream_t);
+template void launch_col_segments<double>(bool, const double*, long long, const double*, long long,
+                                          double*, long long, int, int, int, int, int,
+                                          const Coeffs<double>&, bool, hipStream_t);
+
 template void launch_col_level<float>(bool, const float*, long long, const float*, long long, float*,
                                       long long, int, int, const Coeffs<float>&, bool, hipStream_t);
 template void launch_col_level<double>(bool, const double*, long long, const double*, long long,

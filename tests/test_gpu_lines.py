@@ -122,6 +122,44 @@ def test_contiguous_partial_depth(prec):
                 np.testing.assert_array_equal(t.cpu().numpy(), back, err_msg="inverse")
 
 
+@pytest.mark.parametrize("prec", ["s", "d"])
+def test_contiguous_level0_in_place(prec):
+    """Contiguous signals around the longest signal one workgroup holds whole
+    (level 0 in place; 16384 FP32 / 8192 FP64 samples) and just past it
+    (level 0 through scratch), many short signals per workgroup, and signals
+    whose start or pitch allow only narrower loads."""
+    from oracle.oracle import oracle
+    o = oracle()
+    dt = np.float32 if prec == "s" else np.float64
+    es = np.dtype(dt).itemsize
+    lim = 65536 // es
+    cases_ = [(30, 700, 0, 0), (31, 300, 1, 1), (lim, 5, 0, 0), (lim + 1, 5, 0, 0),
+              (lim - 2, 4, 1, 2), (lim + 3000, 3, 0, 0), (2, 1000, 0, 0)]
+    for N, count, off, extra in cases_:
+        pitch = N + extra
+        sig = workloads.random_walk(count, N, seed=N + count).astype(dt)
+        for w in ("cdf97", "cdf53"):
+            say(f"lines rows {prec} N={N} count={count} offset={off} pitch={pitch} {w}")
+            want_f = oracle_full(o, sig, w, True)
+            want_i = oracle_full(o, want_f, w, False)
+            buf = np.zeros(off + count * pitch, dtype=dt)
+            view = lambda a: a[off:].reshape(count, pitch)[:, :N]
+            view(buf)[:] = sig
+            t = torch.from_numpy(buf).cuda()
+            ptr = t.data_ptr() + off * es
+            torch.cuda.synchronize()
+            j = libdwt_amd.lines_1d(w, True, ptr, pitch * es, es, N, count, -1, prec == "d")
+            torch.cuda.synchronize()
+            got = t.cpu().numpy()
+            np.testing.assert_array_equal(view(got), want_f, err_msg="forward")
+            assert not got[:off].any() and not got[off:].reshape(count, pitch)[:, N:].any()
+            libdwt_amd.lines_1d(w, False, ptr, pitch * es, es, N, count, j, prec == "d")
+            torch.cuda.synchronize()
+            got = t.cpu().numpy()
+            np.testing.assert_array_equal(view(got), want_i, err_msg="inverse")
+            assert not got[:off].any() and not got[off:].reshape(count, pitch)[:, N:].any()
+
+
 def test_slice_digests():
     """The 256 x 8192 random-walk slice as matrix columns through the batched
     entry point, against the digests the reference produced."""
