@@ -131,12 +131,28 @@ template <typename T> struct VecN<T, 1> { typedef T type; };
 template <typename T, int VW>
 __device__ __forceinline__ void res_fill(T* l, int ln, int rows, const T* g1, long long p1, int m1,
                                          const T* g2, long long p2, int m2) {
+  // kFill loads are issued before their LDS stores, so that many of a wave's
+  // loads are in flight at once
+  constexpr int kFill = 8;
   using V = typename VecN<T, VW>::type;
-  const int v1 = m1 / VW, v = v1 + m2 / VW;
-  for (int idx = threadIdx.x; idx < rows * v; idx += blockDim.x) {
-    const int u = idx / v, i = idx - u * v;
-    const T* g = i < v1 ? g1 + u * p1 + (long long)i * VW : g2 + u * p2 + (long long)(i - v1) * VW;
-    *reinterpret_cast<V*>(l + u * ln + i * VW) = *reinterpret_cast<const Gbl<V>*>(gload_ptr(g));
+  const int v1 = m1 / VW, v = v1 + m2 / VW, total = rows * v;
+  for (int base = threadIdx.x; base < total; base += kFill * (int)blockDim.x) {
+    V tmp[kFill];
+    int at[kFill];
+#pragma unroll
+    for (int f = 0; f < kFill; ++f) {
+      const int idx = base + f * (int)blockDim.x;
+      at[f] = -1;
+      if (idx < total) {
+        const int u = idx / v, i = idx - u * v;
+        const T* g = i < v1 ? g1 + u * p1 + (long long)i * VW : g2 + u * p2 + (long long)(i - v1) * VW;
+        tmp[f] = *reinterpret_cast<const Gbl<V>*>(gload_ptr(g));
+        at[f] = u * ln + i * VW;
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < kFill; ++f)
+      if (at[f] >= 0) *reinterpret_cast<V*>(l + at[f]) = tmp[f];
   }
 }
 
@@ -221,6 +237,27 @@ row_res_inv_kernel(const T* ll, long long lp, const T* hh, long long hp, T* dst,
 }
 
 // ---- column kernels (adjacent signals) ---------------------------------------------
+// Stores of a lane's two adjacent columns: one pair-wise store when the row
+// allows it (vec: base pointer and pitch even in elements; uniform).
+template <typename T>
+__device__ __forceinline__ bool pair_ok(const T* p, long long pitch) {
+  return __builtin_amdgcn_readfirstlane(
+      (int)((reinterpret_cast<uintptr_t>(p) % (2 * sizeof(T))) == 0 && (pitch & 1) == 0));
+}
+template <typename T>
+__device__ __forceinline__ void st_pair(T* p, T a, T b, bool va, bool vb, bool vec) {
+  if (vec && vb) {
+    using V2 = typename Vec2<T>::type;
+    V2 v;
+    v.x = a;
+    v.y = b;
+    *gstore_ptr(reinterpret_cast<V2*>(p)) = v;
+  } else {
+    if (va) st_global(p, a);
+    if (vb) st_global(p + 1, b);
+  }
+}
+
 // A signal is a column: sample i of signal x at row i.  Forward level on n
 // rows: src (row pitch sp) -> L rows to ll (pitch llp), H row m to det row
 // hbase + m (pitch dp; hbase = ceil(n/2) puts H in place).  count signals;
@@ -240,6 +277,7 @@ col_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ ll, long
   const int x = (g * kWavesPerBlock + wave) * 128 + 2 * lane;
   if (__builtin_amdgcn_readfirstlane((g * kWavesPerBlock + wave) * 128) >= count) return;
   const bool va = x < count, vb = x + 1 < count;
+  const bool vl = pair_ok(ll, llp), vd = pair_ok(det, dp);
   const int half = (n + 1) >> 1, nh = n >> 1;
   const int m0 = (seg * SS) >> 1, m1 = min(m0 + SS / 2, half);
   const int pbeg = m0 - LAG;
@@ -266,11 +304,9 @@ col_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ ll, long
     cb.push(rb[2 * u], rb[2 * u + 1], cf, Lb, Hb);
     const int m = pbeg + u - LAG;
     if (m >= m0 && m < m1) {
-      if (va) st_global(ll + (long long)m * llp + x, La);
-      if (vb) st_global(ll + (long long)m * llp + x + 1, Lb);
+      st_pair(ll + (long long)m * llp + x, La, Lb, va, vb, vl);
       if (m < nh) {
-        if (va) st_global(det + (long long)(hbase + m) * dp + x, Ha);
-        if (vb) st_global(det + (long long)(hbase + m) * dp + x + 1, Hb);
+        st_pair(det + (long long)(hbase + m) * dp + x, Ha, Hb, va, vb, vd);
       }
     }
   }
@@ -293,6 +329,7 @@ col_inv_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ de
   const int x = (g * kWavesPerBlock + wave) * 128 + 2 * lane;
   if (__builtin_amdgcn_readfirstlane((g * kWavesPerBlock + wave) * 128) >= count) return;
   const bool va = x < count, vb = x + 1 < count;
+  const bool vo = pair_ok(dst, dstp);
   const int half = (n + 1) >> 1;
   const int y0 = seg * SS, y1 = min(y0 + SS, n);
   const int pbeg = (y0 >> 1) - LAG;
@@ -323,18 +360,19 @@ col_inv_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ de
     const int p = pbeg + u;
     const int r0 = NS == 4 ? 2 * p - 3 : 2 * p - 2;
     if (r0 >= y0 && r0 < y1) {
-      if (va) st_global(dst + (long long)r0 * dstp + x, fa);
-      if (vb) st_global(dst + (long long)r0 * dstp + x + 1, fb);
+      st_pair(dst + (long long)r0 * dstp + x, fa, fb, va, vb, vo);
     }
     if (r0 + 1 >= y0 && r0 + 1 < y1) {
-      if (va) st_global(dst + (long long)(r0 + 1) * dstp + x, sa);
-      if (vb) st_global(dst + (long long)(r0 + 1) * dstp + x + 1, sb);
+      st_pair(dst + (long long)(r0 + 1) * dstp + x, sa, sb, va, vb, vo);
     }
   }
 }
 
 // ---- host launchers ------------------------------------------------------------------
-constexpr int kColRows = 8;
+#ifndef DWT_COL_ROWS
+#define DWT_COL_ROWS 8
+#endif
+constexpr int kColRows = DWT_COL_ROWS;  // rows per segment of the column kernels
 
 template <typename T, int NS, bool VEC>
 static void col_go(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c, long long cp,

@@ -56,11 +56,12 @@ def test_lines_match_oracle(layout, prec):
                 t = torch.from_numpy(buf).cuda()
                 args = (t.data_ptr(), (N + 3) * es, es)
                 read = lambda: t.cpu().numpy()[:, :N]
-            elif layout == "cols":  # signal k = column k of an N x (count + 1) matrix
-                buf = np.zeros((N, count + 1), dtype=dt)
+            elif layout == "cols":  # signal k = column k of an N x cp matrix (cp odd for odd N)
+                cp = count + 1 + N % 2
+                buf = np.zeros((N, cp), dtype=dt)
                 buf[:, :count] = sig.T
                 t = torch.from_numpy(buf).cuda()
-                args = (t.data_ptr(), es, (count + 1) * es)
+                args = (t.data_ptr(), es, cp * es)
                 read = lambda: t.cpu().numpy()[:, :count].T
             else:  # samples 3 elements apart, each signal in its own region
                 sg = 3 * N + 1
