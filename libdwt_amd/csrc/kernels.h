@@ -157,7 +157,7 @@ void launch_row_level(bool fwd, const T* a, long long ap, const T* b, long long 
 // The same level on whole rows held in one workgroup's LDS: every row is read
 // before any output is written, so the outputs may overlap the input rows
 // (in place).  Same argument meaning; returns false (nothing launched) when a
-// row of n samples is too long to hold or n < 2.
+// row of n samples is too long to hold or shorter than 64 samples.
 bool row_resident_fits(int n, int elem);
 template <typename T>
 bool launch_row_resident(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c,

@@ -134,14 +134,15 @@ __device__ __forceinline__ void res_fill(T* l, int ln, int rows, const T* g1, lo
   // kFill loads are issued before their LDS stores, so that many of a wave's
   // loads are in flight at once
   constexpr int kFill = 8;
+  constexpr int kThreads = kWave * kWavesPerBlock;  // the kernels' block size
   using V = typename VecN<T, VW>::type;
   const int v1 = m1 / VW, v = v1 + m2 / VW, total = rows * v;
-  for (int base = threadIdx.x; base < total; base += kFill * (int)blockDim.x) {
+  for (int base = threadIdx.x; base < total; base += kFill * kThreads) {
     V tmp[kFill];
     int at[kFill];
 #pragma unroll
     for (int f = 0; f < kFill; ++f) {
-      const int idx = base + f * (int)blockDim.x;
+      const int idx = base + f * kThreads;
       at[f] = -1;
       if (idx < total) {
         const int u = idx / v, i = idx - u * v;
@@ -168,7 +169,9 @@ row_res_fwd_kernel(const T* src, long long sp, T* ll, long long lp, T* hh, long 
   const int rows = min(R, count - r0);
   res_fill<T, VW>(lds, ln, rows, src + (long long)r0 * sp, sp, n, src, 0, 0);
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  // the wave index is uniform: window bookkeeping and row addresses stay scalar
+  const int lane = threadIdx.x & 63, nw = kWavesPerBlock;  // launched with 256 threads
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int half = (n + 1) >> 1, nh = n >> 1;
   const bool core = lane >= NS / 2 && lane < kWave - NS / 2;
   for (int it = wave; it < rows * nstrips; it += nw) {
@@ -183,15 +186,16 @@ row_res_fwd_kernel(const T* src, long long sp, T* ll, long long lp, T* hh, long 
       e = v.x;
       o = v.y;
     } else {
-      e = row[mirror(Xe, n)];
-      o = row[mirror(Xe + 1, n)];
+      e = row[mirror_near(Xe, n)];
+      o = row[mirror_near(Xe + 1, n)];
     }
     row_fwd<T, NS>(e, o, cf);
     const int k = Xe >> 1;
-    const long long r = r0 + u;
+    T* lrow = ll + (long long)(r0 + u) * lp;
+    T* hrow = hh + (long long)(r0 + u) * hp;
     if (core) {
-      if (k < half) st_global(ll + r * lp + k, e);
-      if (k < nh) st_global(hh + r * hp + k, o);
+      if (k < half) st_global(lrow + k, e);
+      if (k < nh) st_global(hrow + k, o);
     }
   }
 }
@@ -211,17 +215,20 @@ row_res_inv_kernel(const T* ll, long long lp, const T* hh, long long hp, T* dst,
   const int rows = min(R, count - r0);
   res_fill<T, VW>(lds, ln, rows, ll + (long long)r0 * lp, lp, half, hh + (long long)r0 * hp, hp, nh);
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  // the wave index is uniform: window bookkeeping and row addresses stay scalar
+  const int lane = threadIdx.x & 63, nw = kWavesPerBlock;  // launched with 256 threads
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const bool core = lane >= NS / 2 && lane < kWave - NS / 2;
   for (int it = wave; it < rows * nstrips; it += nw) {
     const int u = it / nstrips, strip = it - u * nstrips;
     const int Xe = strip * CORE - NS + 2 * lane;
-    const int ke = mirror(Xe, n) >> 1, ko = mirror(Xe + 1, n) >> 1;
+    const int ke = mirror_near(Xe, n) >> 1, ko = mirror_near(Xe + 1, n) >> 1;
     const T* row = lds + u * ln;
     T e = row[ke], o = row[half + ko];
     row_inv<T, NS>(e, o, cf);
     if (core) {
-      T* out = dst + (long long)(r0 + u) * dp + Xe;
+      T* orow = dst + (long long)(r0 + u) * dp;
+      T* out = orow + Xe;
       if (VEC && Xe + 1 < n) {
         using V2 = typename Vec2<T>::type;
         V2 v;
@@ -440,8 +447,10 @@ static void res_width(bool fwd, const T* a, long long ap, const T* b, long long 
   else res_go<T, NS, 1>(fwd, vec, a, ap, b, bp, c, cp, n, count, cf, st);
 }
 
+// Rows shorter than 64 samples: one window each, mostly idle lanes; the
+// windowed kernel through scratch measured faster there (16 and 32 samples).
 bool row_resident_fits(int n, int elem) {
-  return n >= 2 && (long long)((n + 3) & ~3) * elem <= kResMaxBytes;
+  return n >= 64 && (long long)((n + 3) & ~3) * elem <= kResMaxBytes;
 }
 
 template <typename T>

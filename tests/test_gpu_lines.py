@@ -127,15 +127,16 @@ def test_contiguous_partial_depth(prec):
 def test_contiguous_level0_in_place(prec):
     """Contiguous signals around the longest signal one workgroup holds whole
     (level 0 in place; 16384 FP32 / 8192 FP64 samples) and just past it
-    (level 0 through scratch), many short signals per workgroup, and signals
-    whose start or pitch allow only narrower loads."""
+    (level 0 through scratch), many signals of 64 samples per workgroup, the
+    shorter ones (through scratch), and signals whose start or pitch allow
+    only narrower loads."""
     from oracle.oracle import oracle
     o = oracle()
     dt = np.float32 if prec == "s" else np.float64
     es = np.dtype(dt).itemsize
     lim = 65536 // es
-    cases_ = [(30, 700, 0, 0), (31, 300, 1, 1), (lim, 5, 0, 0), (lim + 1, 5, 0, 0),
-              (lim - 2, 4, 1, 2), (lim + 3000, 3, 0, 0), (2, 1000, 0, 0)]
+    cases_ = [(64, 700, 0, 0), (65, 300, 1, 1), (lim, 5, 0, 0), (lim + 1, 5, 0, 0),
+              (lim - 2, 4, 1, 2), (lim + 3000, 3, 0, 0), (2, 1000, 0, 0), (63, 500, 0, 1)]
     for N, count, off, extra in cases_:
         pitch = N + extra
         sig = workloads.random_walk(count, N, seed=N + count).astype(dt)
