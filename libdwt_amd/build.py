@@ -48,7 +48,9 @@ def build(verbose: bool = False, out: str = OUT, defines=()) -> str:
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(lambda s: _compile(s, objdir, defines), SOURCES))
     tmp = out + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs]
+    # --no-undefined: a symbol missing from every object fails the build here
+    # rather than the first dlopen
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-Wl,--no-undefined", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
