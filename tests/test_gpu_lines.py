@@ -81,9 +81,11 @@ def test_lines_match_oracle(layout, prec):
             np.testing.assert_array_equal(read(), want_i, err_msg="inverse")
 
 
+@pytest.mark.parametrize("layout", ["rows", "cols"])
 @pytest.mark.parametrize("prec", ["s", "d"])
-def test_contiguous_partial_depth(prec):
-    """Contiguous signals, 1..4 levels (the last level's L to global memory),
+def test_partial_depth(prec, layout):
+    """Contiguous signals and signals as matrix columns (staged in-place level
+    0 from two levels on), 1..4 levels (the last level's L to global memory),
     against the oracle's caller loop stopped at that depth."""
     from oracle.oracle import oracle
     o = oracle()
@@ -94,7 +96,7 @@ def test_contiguous_partial_depth(prec):
         for w in ("cdf97", "cdf53"):
             for lv in (1, 2, 4):
                 lv = min(lv, workloads.ceil_log2(N))  # the forward clamps to ceil(log2 N)
-                say(f"lines rows {prec} N={N} {w} levels={lv}")
+                say(f"lines {layout} {prec} N={N} {w} levels={lv}")
                 want = sig.copy()
                 for r in want:  # the caller loop, lv levels
                     n = N
@@ -103,14 +105,23 @@ def test_contiguous_partial_depth(prec):
                         r[:len(lo)] = lo
                         r[len(lo):n] = hi
                         n = len(lo)
-                t = torch.from_numpy(sig.copy()).cuda()
+                if layout == "rows":
+                    t = torch.from_numpy(sig.copy()).cuda()
+                    args = (N * es, es)
+                    read = lambda: t.cpu().numpy()
+                else:  # signal k = column k of an N x 10 matrix
+                    t = torch.from_numpy(np.ascontiguousarray(
+                        np.concatenate([sig.T, np.zeros((N, 1), dt)], axis=1))).cuda()
+                    args = (es, 10 * es)
+                    read = lambda: t.cpu().numpy()[:, :9].T
                 torch.cuda.synchronize()
-                j = libdwt_amd.lines_1d(w, True, t.data_ptr(), N * es, es, N, len(sig), lv,
+                j = libdwt_amd.lines_1d(w, True, t.data_ptr(), args[0], args[1], N, len(sig), lv,
                                         prec == "d")
                 torch.cuda.synchronize()
                 assert j == lv
-                np.testing.assert_array_equal(t.cpu().numpy(), want, err_msg="forward")
-                libdwt_amd.lines_1d(w, False, t.data_ptr(), N * es, es, N, len(sig), lv, prec == "d")
+                np.testing.assert_array_equal(read(), want, err_msg="forward")
+                libdwt_amd.lines_1d(w, False, t.data_ptr(), args[0], args[1], N, len(sig), lv,
+                                    prec == "d")
                 torch.cuda.synchronize()
                 back = want.copy()
                 sizes = [N]
@@ -120,7 +131,7 @@ def test_contiguous_partial_depth(prec):
                     for n in reversed(sizes):
                         h = (n + 1) // 2
                         r[:n] = o.i_line(r[:h].copy(), r[h:n].copy(), w)
-                np.testing.assert_array_equal(t.cpu().numpy(), back, err_msg="inverse")
+                np.testing.assert_array_equal(read(), back, err_msg="inverse")
 
 
 @pytest.mark.parametrize("prec", ["s", "d"])
