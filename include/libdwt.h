@@ -367,10 +367,12 @@ long long dwt_gpu_pyr_trace_read(void *stream, unsigned long long *out, long lon
  * to [0, ceil(n/2)) and [ceil(n/2), n) of each signal for n = length,
  * ceil(length/2), ... while n >= 2; the inverse undoes it.  *levels: forward,
  * requested (< 0 or too large: full depth) and written back; inverse, the
- * levels to undo.  Asynchronous on `stream`.  Fast paths: contiguous samples
- * (length <= 16384 FP32 / 8192 FP64), and adjacent signals (signal_stride =
- * element size: the columns of a matrix); any other layout runs one launch
- * per level. */
+ * levels to undo.  Asynchronous on `stream`.  Fast paths, one launch per
+ * level: contiguous samples (level 0 in place for 64 <= length <= 16384
+ * FP32 / 8192 FP64, through scratch otherwise), and adjacent signals
+ * (signal_stride = element size: the columns of a matrix; level 0 in place
+ * but for its top eighth).  Any other layout runs one generic launch per
+ * level. */
 int dwt_gpu_1d_batch(int wavelet, int direction, int precision, void *ptr, long long signal_stride,
                      long long sample_stride, int length, int count, int *levels, void *stream);
 
