@@ -473,6 +473,23 @@ def bench_lines(args, torch, libdwt_amd, workloads):
                        "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                        "max_abs_roundtrip_err": err}
         del t
+    # ceiling: a device copy of the same 512 MiB matrix (it does not fit the
+    # 256 MB Infinity Cache, unlike the 64 MiB 2D benchmark image)
+    ca = torch.empty(count * N, dtype=torch.float32, device=dev)
+    cb = torch.empty_like(ca)
+    with torch.cuda.stream(stream):
+        for _ in range(2):
+            cb.copy_(ca)
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record(stream)
+        for _ in range(10):
+            cb.copy_(ca)
+        c1.record(stream)
+    torch.cuda.synchronize()
+    copy_gbs = 8 * count * N / (c0.elapsed_time(c1) * 1e-3 / 10) / 1e9
+    del ca, cb
+    for r in res.values():
+        r["frac_of_copy"] = round(r["GB_s"] / copy_gbs, 4)
     out = {
         "metric": f"Msamples/s, 16384 x 8192 random walks, full-depth 1D {wv} fwd+inv",
         "value": res["interleaved"]["Msamples_s"], "unit": "Msamples/s",
@@ -483,6 +500,7 @@ def bench_lines(args, torch, libdwt_amd, workloads):
         "config": {"workload": "lines", "signals": count, "length": N, "levels": levels,
                    "wavelet": wv},
         "layouts": res,
+        "copy_ceiling": {"gbs": round(copy_gbs, 1), "bytes": 8 * count * N},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_lines_baseline(sig, wv, args.cpu_seconds)
