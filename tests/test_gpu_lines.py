@@ -173,6 +173,31 @@ def test_contiguous_level0_in_place(prec):
             assert not got[:off].any() and not got[off:].reshape(count, pitch)[:, N:].any()
 
 
+def test_degenerate_arguments():
+    """Length 0 and 1 (no level) and no signals touch nothing; an inverse
+    asked for more levels than the length allows, and negative sizes, are
+    errors that leave the data as it was."""
+    sig = torch.arange(64 * 5, dtype=torch.float32, device="cuda").reshape(5, 64)
+    keep = sig.clone()
+    for N, count in ((0, 5), (1, 5), (64, 0)):
+        for layout in ("rows", "cols"):
+            args = (64 * 4, 4) if layout == "rows" else (4, 64 * 4)
+            say(f"degenerate {layout} N={N} count={count}")
+            j = libdwt_amd.lines_1d("cdf97", True, sig.data_ptr(), args[0], args[1], N, count, -1)
+            assert j == workloads.ceil_log2(max(N, 1))  # the depth, also with no signals
+            libdwt_amd.lines_1d("cdf97", False, sig.data_ptr(), args[0], args[1], N, count, j)
+    torch.cuda.synchronize()
+    assert torch.equal(sig, keep)
+    for bad in (dict(N=64, count=5, lv=7, fwd=False), dict(N=-1, count=5, lv=-1, fwd=True),
+                dict(N=64, count=-2, lv=-1, fwd=True)):
+        say(f"error case {bad}")
+        with pytest.raises(RuntimeError):
+            libdwt_amd.lines_1d("cdf97", bad["fwd"], sig.data_ptr(), 64 * 4, 4, bad["N"], bad["count"],
+                                bad["lv"])
+    torch.cuda.synchronize()
+    assert torch.equal(sig, keep)
+
+
 def test_slice_digests():
     """The 256 x 8192 random-walk slice as matrix columns through the batched
     entry point, against the digests the reference produced."""
