@@ -79,8 +79,9 @@ def bench_sparse(args, torch, libdwt_amd, workloads) -> None:
 
 def bench_channels(args, torch, libdwt_amd, workloads) -> None:
     """A 3-channel interleaved FP32 image (the layout the reference's OpenCV
-    wrapper passes per channel: stride_y = 3 floats), every channel fwd+inv
-    through the drop-in drivers' arguments; against three planar images."""
+    wrapper passes per channel: stride_y = 3 floats), every channel fwd+inv:
+    through the drop-in drivers' arguments one channel at a time, through
+    dwt_gpu_2d_channels (all channels per call), and as three planar images."""
     W, H, J, wv = args.width, args.height, args.levels, args.wavelet
     img = torch.stack([torch.from_numpy(workloads.sinusoid_image(W, H, seed=3 + c)) for c in range(3)],
                       dim=-1).cuda().contiguous()  # H x W x 3
@@ -99,7 +100,11 @@ def bench_channels(args, torch, libdwt_amd, workloads) -> None:
         for p in planes:
             j = libdwt_amd.transform_2d_async(wv, True, p.data_ptr(), 4 * W, 4, W, H, j=J, stream=sp)
             libdwt_amd.transform_2d_async(wv, False, p.data_ptr(), 4 * W, 4, W, H, j=j, stream=sp)
-    for name, fn in (("interleaved", inter), ("planar", planar)):
+
+    def multi():  # dwt_gpu_2d_channels: all three channels in one call per direction
+        j = libdwt_amd.transform_channels(wv, True, img.data_ptr(), 12 * W, 12, 4, 3, W, H, j=J, stream=sp)
+        libdwt_amd.transform_channels(wv, False, img.data_ptr(), 12 * W, 12, 4, 3, W, H, j=j, stream=sp)
+    for name, fn in (("interleaved", inter), ("interleaved_one_call", multi), ("planar", planar)):
         for _ in range(args.warmup):
             fn()
         ms = _timed(torch, stream, fn, args.steps)

@@ -251,6 +251,19 @@ int dwt_gpu_2d_async(int wavelet, int direction, int precision, void *ptr, int s
                      int size_i_big_y, int *j, int decompose_one, int zero_padding,
                      void *stream);
 
+/* Every channel of a multi-channel image in one call.  Channel c is the
+ * size_x x size_y frame at ptr + c * channel_stride (bytes), rows stride_x
+ * and pixels stride_y bytes apart, as the drivers take them (outer frame =
+ * inner frame); the channels must not overlap.  Each channel is transformed
+ * exactly as dwt_gpu_2d_async on that frame would (same *j semantics,
+ * decompose_one, zero_padding).  No counterpart in the reference: it replaces
+ * the per-channel driver calls of its OpenCV wrapper (src/cvdwt.cpp:67-147)
+ * for interleaved images, gathering the channels into packed planes in one
+ * pass and scattering them back in one pass.  Asynchronous on `stream`. */
+int dwt_gpu_2d_channels(int wavelet, int direction, int precision, void *ptr, int stride_x,
+                        int stride_y, int channel_stride, int channels, int size_x, int size_y,
+                        int *j, int decompose_one, int zero_padding, void *stream);
+
 /* Accuracy statistics of a device-side compare (dwt_gpu_compare). */
 #define DWT_GPU_ULP_BINS 32
 typedef struct dwt_gpu_compare_stats {

@@ -99,6 +99,8 @@ def lib() -> C.CDLL:
         L.dwt_gpu_version.restype = C.c_char_p
         L.dwt_gpu_2d_async.argtypes = [i, i, i, vp, i, i, i, i, i, i, C.POINTER(i), i, i, vp]
         L.dwt_gpu_2d_async.restype = i
+        L.dwt_gpu_2d_channels.argtypes = [i, i, i, vp, i, i, i, i, i, i, C.POINTER(i), i, i, vp]
+        L.dwt_gpu_2d_channels.restype = i
         L.dwt_gpu_profile_enable.argtypes = [i]
         L.dwt_gpu_profile_enable.restype = i
         L.dwt_gpu_profile_read.argtypes = [i, i, C.POINTER(C.c_longlong), C.POINTER(C.c_double)]
@@ -191,6 +193,21 @@ def transform_2d_async(wavelet: str, forward: bool, ptr: int, stride_x: int, str
                                 size_o_x if size_i_x is None else size_i_x,
                                 size_o_y if size_i_y is None else size_i_y, C.byref(jj),
                                 decompose_one, zero_padding, stream)
+    _check(rc)
+    return jj.value
+
+
+def transform_channels(wavelet: str, forward: bool, ptr: int, stride_x: int, stride_y: int,
+                       channel_stride: int, channels: int, size_x: int, size_y: int, j: int = -1,
+                       decompose_one: int = 0, zero_padding: int = 0, f64: bool = False,
+                       stream: int = 0) -> int:
+    """dwt_gpu_2d_channels: every channel of a multi-channel image on device
+    memory (channel c at ptr + c * channel_stride, driver strides in bytes);
+    returns the (clamped) level count."""
+    jj = C.c_int(j)
+    rc = lib().dwt_gpu_2d_channels(WAVELET_IDS[wavelet], FORWARD if forward else INVERSE,
+                                   F64 if f64 else F32, ptr, stride_x, stride_y, channel_stride, channels,
+                                   size_x, size_y, C.byref(jj), decompose_one, zero_padding, stream)
     _check(rc)
     return jj.value
 

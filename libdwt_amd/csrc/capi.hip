@@ -911,6 +911,47 @@ static void inv2_device(char* ptr, long long sx, long long sy, int Wo, int Ho, i
   if (jf > 0) fast_inv<T>(reinterpret_cast<T*>(ptr), sx / (long long)sizeof(T), Wo, Ho, jf, cf, sc, st);
 }
 
+// Every channel of a multi-channel image (channel c: the W x H frame at
+// ptr + c * sc bytes, strides sx / sy as the drivers take them): the channels
+// are gathered into packed planes in one pass, each plane is transformed on
+// the packed path as the drivers would transform that channel alone, and the
+// planes are scattered back in one pass.  (Per-channel driver calls, as the
+// reference's OpenCV wrapper makes them, src/cvdwt.cpp:67-147, restage each
+// channel separately and so read every pixel's whole group of channels once
+// per channel.)
+template <typename T>
+static void run_channels(bool fwd, char* ptr, long long sx, long long sy, long long sc, int C, int W, int H,
+                         int* jp, int d1, int zp, Wavelet w, hipStream_t st) {
+  init_once();
+  const long long es = sizeof(T);
+  if (C < 1 || W < 0 || H < 0) fail(DWT_GPU_EINVAL, "channels %d, size %d x %d", C, W, H);
+  if (sx % es || sy % es || sc % es || reinterpret_cast<uintptr_t>(ptr) % es)
+    fail(DWT_GPU_EINVAL, "pointer and strides must be whole elements");
+  const int jl = level_limit(W, H, d1);
+  int J;
+  if (fwd) {
+    J = (*jp < 0 || *jp > jl) ? jl : *jp;  // src/libdwt.c:6326-6329
+    *jp = J;
+  } else {
+    J = (*jp >= 0 && *jp < jl) ? *jp : jl;  // src/libdwt.c:6678-6681
+  }
+  if (J == 0 || W == 0 || H == 0) return;
+  const long long pp = (long long)pad_pitch(W), step = pp * H;
+  T* planes = (T*)scratch_for(st).get(8, sizeof(T) * (size_t)step * C);
+  T* img = reinterpret_cast<T*>(ptr);
+  LAUNCH(launch_channels<T>(true, img, sx / es, sy / es, sc / es, C, W, H, planes, pp, step, st));
+  for (int c = 0; c < C; ++c) {
+    char* plane = reinterpret_cast<char*>(planes + c * step);
+    if (fwd) {
+      int j = J;
+      fwd2_device<T>(plane, pp * es, es, W, H, W, H, &j, d1, zp, w, st);
+    } else {
+      inv2_device<T>(plane, pp * es, es, W, H, W, H, J, d1, zp, w, st);
+    }
+  }
+  LAUNCH(launch_channels<T>(false, img, sx / es, sy / es, sc / es, C, W, H, planes, pp, step, st));
+}
+
 // ---- pointer classification and staging --------------------------------------------
 // Returns the device-usable address of p, or nullptr for pageable host memory.
 static void* device_address(const void* p) {
@@ -1439,6 +1480,33 @@ int dwt_gpu_2d_async(int wavelet, int direction, int precision, void* ptr, int s
     else
       run2d<double>(fwd, dptr, stride_x, stride_y, size_o_big_x, size_o_big_y, size_i_big_x,
                     size_i_big_y, j, decompose_one, zero_padding, w, st, false);
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_2d_channels(int wavelet, int direction, int precision, void* ptr, int stride_x, int stride_y,
+                        int channel_stride, int channels, int size_x, int size_y, int* j, int decompose_one,
+                        int zero_padding, void* stream) {
+  try {
+    g_err.clear();
+    if (wavelet != DWT_GPU_CDF97 && wavelet != DWT_GPU_CDF53) fail(DWT_GPU_EINVAL, "bad wavelet");
+    if (direction != DWT_GPU_FORWARD && direction != DWT_GPU_INVERSE) fail(DWT_GPU_EINVAL, "bad direction");
+    if (precision != DWT_GPU_F32 && precision != DWT_GPU_F64) fail(DWT_GPU_EINVAL, "bad precision");
+    if (!j) fail(DWT_GPU_EINVAL, "null j");
+    if (!ptr) fail(DWT_GPU_EINVAL, "null image pointer");
+    void* dptr = device_address(ptr);
+    if (!dptr) fail(DWT_GPU_EMEMORY, "image pointer is not device-accessible memory");
+    const Wavelet w = wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53;
+    const bool fwd = direction == DWT_GPU_FORWARD;
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == DWT_GPU_F32)
+      run_channels<float>(fwd, (char*)dptr, stride_x, stride_y, channel_stride, channels, size_x, size_y, j,
+                          decompose_one, zero_padding, w, st);
+    else
+      run_channels<double>(fwd, (char*)dptr, stride_x, stride_y, channel_stride, channels, size_x, size_y, j,
+                           decompose_one, zero_padding, w, st);
     return DWT_GPU_OK;
   } catch (const Failure& f) {
     return f.code;

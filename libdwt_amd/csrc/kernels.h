@@ -90,6 +90,13 @@ void launch_to_f64(const void* src, long long sx, long long sy, double* dst, int
 template <typename T>
 void launch_copy_strided(const T* src, long long sr, long long sc, T* dst, long long dr,
                          long long dc, int W, int H, hipStream_t st);
+// The C channels of a W x H image (element strides: rows sr, pixels sx,
+// channels sc) to (gather) or from C packed planes, plane c at
+// planes + c * plane_step with row pitch pp; every pixel's channels in one
+// pass.
+template <typename T>
+void launch_channels(bool gather, T* img, long long sr, long long sx, long long sc, int C, int W, int H,
+                     T* planes, long long pp, long long plane_step, hipStream_t st);
 // Copy a w x h rectangle between packed buffers.
 template <typename T>
 void launch_copy_rect(const T* src, long long sp, T* dst, long long dp, int w, int h,

@@ -153,6 +153,26 @@ __global__ void copy_rect_kernel(const T* __restrict__ src, long long sp, T* __r
   }
 }
 
+// The C channels of a W x H image (element strides: rows sr, pixels sx,
+// channels sc) to / from C packed planes (plane c at planes + c * plane_step,
+// row pitch pp).  One thread per pixel moves all of its channels, so an
+// interleaved image is read (or written) once, in order.
+template <typename T, bool GATHER>
+__global__ void channels_kernel(T* __restrict__ img, long long sr, long long sx, long long sc, int C, int W,
+                                int H, T* __restrict__ planes, long long pp, long long plane_step) {
+  const long long n = (long long)W * H;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long y = t / W, x = t - y * W;
+    T* px = img + y * sr + x * sx;
+    T* pl = planes + y * pp + x;
+    for (int c = 0; c < C; ++c) {
+      if (GATHER) st_global(pl + c * plane_step, px[c * sc]);
+      else st_global(px + c * sc, pl[c * plane_step]);
+    }
+  }
+}
+
 // Copy of a W x H frame between two element-strided layouts (row stride r,
 // column stride c, in elements, either sign).  The index order follows the
 // source's shorter stride when it gathers (xfast: x varies fastest), so one
@@ -423,6 +443,26 @@ template void launch_copy_strided<float>(const float*, long long, long long, flo
                                          long long, int, int, hipStream_t);
 template void launch_copy_strided<double>(const double*, long long, long long, double*, long long,
                                           long long, int, int, hipStream_t);
+
+template <typename T>
+void launch_channels(bool gather, T* img, long long sr, long long sx, long long sc, int C, int W, int H,
+                     T* planes, long long pp, long long plane_step, hipStream_t st) {
+  const long long n = (long long)W * H;
+  if (n <= 0 || C <= 0) return;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (gather)
+    hipLaunchKernelGGL((channels_kernel<T, true>), dim3((unsigned)blocks), dim3(256), 0, st, img, sr, sx, sc,
+                       C, W, H, planes, pp, plane_step);
+  else
+    hipLaunchKernelGGL((channels_kernel<T, false>), dim3((unsigned)blocks), dim3(256), 0, st, img, sr, sx,
+                       sc, C, W, H, planes, pp, plane_step);
+}
+
+template void launch_channels<float>(bool, float*, long long, long long, long long, int, int, int, float*,
+                                     long long, long long, hipStream_t);
+template void launch_channels<double>(bool, double*, long long, long long, long long, int, int, int,
+                                      double*, long long, long long, hipStream_t);
 template void launch_fwd_level<float>(const float*, long long, int, int, float*, long long,
                                       float*, long long, const Coeffs<float>&, hipStream_t);
 template void launch_fwd_level<double>(const double*, long long, int, int, double*, long long,
