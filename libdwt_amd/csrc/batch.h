@@ -45,7 +45,8 @@ struct BatchImage {
   int W, H, levels, block0;
 };
 // The max |x| pass: a block takes maxabs_rows(W) whole rows of one image
-// (about 16 pixels per thread), so no index needs a division.
+// (about 16 pixels per thread per pass; one row, in passes of kMaxAbsPixels,
+// when a row is longer), so no index needs a division.
 constexpr int kMaxAbsPixels = 256 * 16;
 __host__ __device__ inline int maxabs_rows(int W) {
   return W >= kMaxAbsPixels ? 1 : kMaxAbsPixels / (W > 0 ? W : 1);

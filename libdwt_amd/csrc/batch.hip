@@ -184,28 +184,62 @@ batch_maxabs_kernel(const BatchImage* __restrict__ imgs, const int* __restrict__
   const int y0 = (blockIdx.x - im.block0) * R;
   const int n = min(R, im.H - y0) * W;  // pixels of this block's rows
   const Gbl<float>* src = gload_ptr(static_cast<const float*>(im.src)) + (long long)y0 * im.sp;
-  // (y, x) of pixel e advance with e by one subtraction loop instead of a
-  // division; all of a thread's loads are issued before any is reduced (one
-  // memory round trip per block)
-  constexpr int K = kMaxAbsPixels / 256;
-  int e = threadIdx.x, y = e / W, x = e - y * W;
-  long long off[K];
+  float m = 0.0f;
+  const bool v4 = (W & 3) == 0 && (im.sp & 3) == 0 && (reinterpret_cast<uintptr_t>(im.src) & 15) == 0;
+  // kMaxAbsPixels per pass; a block has more pixels only when it holds one
+  // row longer than that
+  for (int c0 = 0; c0 < n; c0 += kMaxAbsPixels) {
+    const int lim = min(n, c0 + kMaxAbsPixels);
+    if (v4) {
+      // rows of whole 16-byte groups: four pixels per load, 32-bit offsets
+      // from the block's first row
+      using F4 = float __attribute__((ext_vector_type(4)));
+      constexpr int K4 = kMaxAbsPixels / 256 / 4;
+      const int W4 = W >> 2, lim4 = lim >> 2;
+      const int sp4 = (int)(im.sp >> 2);
+      const Gbl<F4>* src4 = reinterpret_cast<const Gbl<F4>*>(src);
+      int e = (c0 >> 2) + threadIdx.x, y = e / W4, x = e - y * W4;
+      int off[K4];
 #pragma unroll
-  for (int u = 0; u < K; ++u) {
-    off[u] = e < n ? (long long)y * im.sp + x : -1;
-    e += 256;
-    x += 256;
-    while (x >= W) {
-      x -= W;
-      ++y;
+      for (int u = 0; u < K4; ++u) {
+        off[u] = e < lim4 ? y * sp4 + x : -1;
+        e += 256;
+        x += 256;
+        while (x >= W4) {
+          x -= W4;
+          ++y;
+        }
+      }
+      F4 v[K4];
+#pragma unroll
+      for (int u = 0; u < K4; ++u) v[u] = off[u] >= 0 ? src4[off[u]] : F4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int u = 0; u < K4; ++u)
+        m = fmaxf(m, fmaxf(fmaxf(fabsf(v[u].x), fabsf(v[u].y)), fmaxf(fabsf(v[u].z), fabsf(v[u].w))));
+    } else {
+      // (y, x) of pixel e advance with e by one subtraction loop instead of
+      // a division; all of a thread's loads are issued before any is reduced
+      // (one memory round trip per pass)
+      constexpr int K = kMaxAbsPixels / 256;
+      int e = c0 + threadIdx.x, y = e / W, x = e - y * W;
+      long long off[K];
+#pragma unroll
+      for (int u = 0; u < K; ++u) {
+        off[u] = e < lim ? (long long)y * im.sp + x : -1;
+        e += 256;
+        x += 256;
+        while (x >= W) {
+          x -= W;
+          ++y;
+        }
+      }
+      float v[K];
+#pragma unroll
+      for (int u = 0; u < K; ++u) v[u] = off[u] >= 0 ? fabsf(src[off[u]]) : 0.0f;
+#pragma unroll
+      for (int u = 0; u < K; ++u) m = fmaxf(m, v[u]);
     }
   }
-  float v[K];
-#pragma unroll
-  for (int u = 0; u < K; ++u) v[u] = off[u] >= 0 ? fabsf(src[off[u]]) : 0.0f;
-  float m = 0.0f;
-#pragma unroll
-  for (int u = 0; u < K; ++u) m = fmaxf(m, v[u]);
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = __float_as_uint(m);
   __syncthreads();

@@ -73,11 +73,10 @@ def test_batch_f32_matches_single_image_oracle(wavelet):
         np.testing.assert_array_equal(out.cpu().numpy(), want, err_msg=f"inverse {x.shape} J={j}")
 
 
-@pytest.mark.parametrize("storage", ["f16", "bf16"])
-@pytest.mark.parametrize("wavelet", ["cdf97", "cdf53"])
-def test_batch_16bit_storage(wavelet, storage):
-    imgs = images(9)
-    say(f"batch {storage} {wavelet}")
+def check_16bit(imgs, wavelet, storage):
+    """Forward in FP32 and in 16-bit storage: every scale a power of two,
+    every stored coefficient within 2^14 after scaling and the FP32 one
+    scaled and rounded to nearest; the inverse within RECON_TOL of max |x|."""
     ts = [torch.from_numpy(x).cuda() for x in imgs]
     c32, _, done = libdwt_amd.batch_forward(ts, wavelet, "f32", -1)
     c16, scales, done16 = libdwt_amd.batch_forward(ts, wavelet, storage, -1)
@@ -102,7 +101,7 @@ def test_batch_16bit_storage(wavelet, storage):
         assert not np.isnan(smap).any()
         c = a.cpu().numpy().astype(np.float64)
         q = b.float().cpu().numpy().astype(np.float64)
-        assert np.abs(c * smap).max() <= 2.0 ** 14 * (1 + 2.0 ** -mant)
+        assert np.abs(c * smap).max() <= 2.0 ** 14 * (1 + 2.0 ** -mant), x.shape
         # round to nearest: error of the scaled value <= half an ulp (or half
         # the FP16 subnormal spacing)
         err = np.abs(q - c * smap)
@@ -114,6 +113,28 @@ def test_batch_16bit_storage(wavelet, storage):
         e = float(np.abs(out.cpu().numpy() - x).max()) / float(np.abs(x).max())
         say(f"  {x.shape} relative reconstruction error {e:.2e}")
         assert e <= RECON_TOL[storage]
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("wavelet", ["cdf97", "cdf53"])
+def test_batch_16bit_storage(wavelet, storage):
+    say(f"batch {storage} {wavelet}")
+    check_16bit(images(9), wavelet, storage)
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_batch_16bit_rows_longer_than_one_pass(storage):
+    """Rows longer than one max |x| pass (4096 pixels), with the largest value
+    past the first pass: the scales must still see it.  Widths with and
+    without whole 16-byte groups."""
+    rng = np.random.default_rng(13)
+    imgs = []
+    for W, H in ((5000, 6), (4100, 3), (4097, 2), (8200, 2), (4096, 2)):
+        x = rng.standard_normal((H, W)).astype(np.float32)
+        x[H // 2, W - 3] = 3.0e4
+        imgs.append(x)
+    say(f"batch {storage} long rows")
+    check_16bit(imgs, "cdf97", storage)
 
 
 def test_batch_plan_replays_and_graph_capture():
