@@ -278,7 +278,10 @@ col_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ ll, long
                Coeffs<T> cf) {
   constexpr int LAG = NS / 2, NP = SS / 2 + NS;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int g = b / nseg, seg = seg0 + (b - g * nseg);
+  // row-major: consecutive workgroups take adjacent column groups of one row
+  // segment, so the workgroups in flight read whole rows
+  const int ngr = (count + 128 * kWavesPerBlock - 1) / (128 * kWavesPerBlock);
+  const int sl = b / ngr, g = b - sl * ngr, seg = seg0 + sl;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int x = (g * kWavesPerBlock + wave) * 128 + 2 * lane;
@@ -330,7 +333,10 @@ col_inv_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ de
                Coeffs<T> cf) {
   constexpr int LAG = NS / 2, NP = SS / 2 + NS;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int g = b / nseg, seg = seg0 + (b - g * nseg);
+  // row-major: consecutive workgroups take adjacent column groups of one row
+  // segment, so the workgroups in flight read whole rows
+  const int ngr = (count + 128 * kWavesPerBlock - 1) / (128 * kWavesPerBlock);
+  const int sl = b / ngr, g = b - sl * ngr, seg = seg0 + sl;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int x = (g * kWavesPerBlock + wave) * 128 + 2 * lane;
