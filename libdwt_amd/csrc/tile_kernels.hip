@@ -9,13 +9,30 @@
 namespace dwt {
 
 // ---- per-level kernels (many workgroups) -------------------------------------
+// Tiles are numbered row-major (ngroups column groups per row segment).  For
+// the forward level the XCDs take bands of kXcdBandSegs row segments in turn,
+// so neighbouring row segments (which share halo rows) mostly run on one
+// XCD's L2 while all XCDs work on nearby rows; grids that are not a whole
+// number of rounds keep one contiguous run per XCD (xcd_remap), as the
+// inverse level always does (bands measured slower there).
+#ifndef DWT_XCD_BAND_SEGS
+#define DWT_XCD_BAND_SEGS 16
+#endif
+constexpr int kXcdBandSegs = DWT_XCD_BAND_SEGS;
+__device__ inline int xcd_band_remap(int b, int nb, int ngroups) {
+  const int C = kXcdBandSegs * ngroups;
+  if (nb % (8 * C) != 0) return xcd_remap(b, nb);
+  const int x = b & 7, idx = b >> 3;
+  return (x + 8 * (idx / C)) * C + idx % C;
+}
+
 template <typename T, int NS, bool VEC, int U>
 __global__ void __launch_bounds__(256)
 fwd_level_kernel(const T* __restrict__ src, long long sp, int W, int H, T* __restrict__ ll,
                  long long llp, T* __restrict__ det, long long dp, int S, int nstrips, int nseg,
                  Coeffs<T> cf) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int ngroups = (nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int b = xcd_band_remap(blockIdx.x, gridDim.x, ngroups);
   const int s = b / ngroups, g = b - s * ngroups;
   const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
@@ -28,8 +45,8 @@ __global__ void __launch_bounds__(256)
 inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ det, long long dp,
                  T* __restrict__ dst, long long dstp, int W, int H, int S, int nstrips, int nseg,
                  Coeffs<T> cf) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int ngroups = (nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);  // bands measured slower for the inverse
   const int s = b / ngroups, g = b - s * ngroups;
   const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
