@@ -106,11 +106,25 @@ struct OutRt {
 };
 
 // ---- level kernels ------------------------------------------------------------------
+// Workgroup order of the batched level kernels: the hardware deals
+// consecutive workgroups to the 8 XCDs round-robin; here each XCD takes
+// kBatchBand consecutive logical workgroups in turn (the tail of a grid that
+// is not a whole number of rounds keeps the plain order).  Measured on 256
+// uneven images against plain order (1), 4, 16, 32, 64 and 128 and one
+// contiguous run per XCD: 8 was fastest (FP32 step 0.645 vs 0.680 ms).
+constexpr int kBatchBand = 8;
+__device__ inline int batch_band_remap(int b, int nb) {
+  const int full = (nb / (8 * kBatchBand)) * (8 * kBatchBand);
+  if (b >= full) return b;
+  const int x = b & 7, idx = b >> 3;
+  return (x + 8 * (idx / kBatchBand)) * kBatchBand + idx % kBatchBand;
+}
+
 template <int NS, int SK, int SS>
 __global__ void __launch_bounds__(256)
 batch_fwd_kernel(const BatchItem* __restrict__ items, const int* __restrict__ block_item,
                  Coeffs<float> cf) {
-  const int b = blockIdx.x;
+  const int b = batch_band_remap(blockIdx.x, gridDim.x);
   const BatchItem& e = items[block_item[b]];
   const int local = b - e.block0;
   const int s = local / e.ngroups, g = local - s * e.ngroups;
@@ -141,7 +155,7 @@ template <int NS, int SK, int SS>
 __global__ void __launch_bounds__(256)
 batch_inv_kernel(const BatchItem* __restrict__ items, const int* __restrict__ block_item,
                  Coeffs<float> cf) {
-  const int b = blockIdx.x;
+  const int b = batch_band_remap(blockIdx.x, gridDim.x);
   const BatchItem& e = items[block_item[b]];
   const int local = b - e.block0;
   const int s = local / e.ngroups, g = local - s * e.ngroups;
