@@ -685,7 +685,7 @@ def main():
     mpix = W * H / (ms_per_step * 1e-3) / 1e6
     alg_step = workloads.algorithmic_bytes(W, H, J) * 2
     out = {
-        "metric": "Mpix/s, 4096^2 CDF9/7 J=8 fwd+inv",
+        "metric": f"Mpix/s, {W}x{H} {'CDF9/7' if wv == 'cdf97' else 'CDF5/3'} J={J} fwd+inv",
         "value": round(mpix, 3),
         "unit": "Mpix/s",
         "n_gpus": 1,

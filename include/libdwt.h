@@ -467,7 +467,12 @@ int dwt_gpu_server_stats(void *server, long long *batches, long long *requests);
 /* Waits for the requests in flight, then frees the server. */
 int dwt_gpu_server_destroy(void *server);
 
-/* Bytes of device scratch the calling thread currently holds for `stream`. *//* Bytes of device scratch the calling thread currently holds for `stream`. */
+/* Bytes of device scratch the calling thread currently holds for `stream`.
+ * Scratch is kept per (host thread, stream, device) and reused by later calls;
+ * a thread's sets are freed when the thread exits (except the thread that
+ * loaded the library, whose sets live until the process ends) or by
+ * dwt_gpu_release_scratch.  A caller that destroys a stream should release
+ * its scratch first. */
 long long dwt_gpu_scratch_bytes(void *stream);
 /* Release the calling thread's scratch for `stream` (synchronises it). */
 int dwt_gpu_release_scratch(void *stream);

@@ -266,16 +266,26 @@ static void batch_plan_execute(const BatchPlan& pl, hipStream_t st) {
   pl.t.launch(static_cast<char*>(pl.dev), st);
 }
 
-// dwt_gpu_batch_2d keeps the plans of its most recent batches.
+// dwt_gpu_batch_2d keeps the plans of its most recent batches.  A plan is
+// held by shared ownership: a caller keeps its reference until its launches
+// are enqueued, so another thread that evicts the plan meanwhile only drops
+// the cache's reference.  The last owner frees it, after a device
+// synchronisation (its tables may still be read by launches on any stream).
+using PlanRef = std::shared_ptr<BatchPlan>;
+static void release_plan(BatchPlan* p) {
+  hipDeviceSynchronize();
+  delete p;
+}
 static std::mutex g_batch_mu;
-static std::vector<BatchPlan*>* g_batch_cache = new std::vector<BatchPlan*>();
+static std::vector<PlanRef>* g_batch_cache = new std::vector<PlanRef>();
 static constexpr size_t kBatchCache = 4;
 
-static BatchPlan* batch_plan_cached(int wavelet, int direction, int storage, dwt_gpu_image* images,
-                                    int n, hipStream_t st) {
+static PlanRef batch_plan_cached(int wavelet, int direction, int storage, dwt_gpu_image* images, int n,
+                                 hipStream_t st) {
+  PlanRef evicted;  // released after the lock, by whichever owner is last
   std::lock_guard<std::mutex> lk(g_batch_mu);
   for (size_t i = 0; i < g_batch_cache->size(); ++i) {
-    BatchPlan* p = (*g_batch_cache)[i];
+    PlanRef p = (*g_batch_cache)[i];
     if (p->t.wavelet != wavelet || p->t.fwd != (direction == DWT_GPU_FORWARD) ||
         p->t.storage != storage || p->t.n != n)
       continue;
@@ -292,11 +302,10 @@ static BatchPlan* batch_plan_cached(int wavelet, int direction, int storage, dwt
     g_batch_cache->insert(g_batch_cache->begin(), p);
     return p;
   }
-  BatchPlan* p = batch_plan_create(wavelet, direction, storage, images, n, st);
+  PlanRef p(batch_plan_create(wavelet, direction, storage, images, n, st), release_plan);
   g_batch_cache->insert(g_batch_cache->begin(), p);
   if (g_batch_cache->size() > kBatchCache) {
-    HIPCHK(hipDeviceSynchronize());  // the evicted plan may still be in use on some stream
-    delete g_batch_cache->back();
+    evicted = std::move(g_batch_cache->back());
     g_batch_cache->pop_back();
   }
   return p;
@@ -343,7 +352,7 @@ int dwt_gpu_batch_2d(int wavelet, int direction, int storage, dwt_gpu_image* ima
                      void* stream) {
   try {
     g_err.clear();
-    BatchPlan* p = batch_plan_cached(wavelet, direction, storage, images, n, (hipStream_t)stream);
+    const PlanRef p = batch_plan_cached(wavelet, direction, storage, images, n, (hipStream_t)stream);
     batch_plan_execute(*p, (hipStream_t)stream);
     return DWT_GPU_OK;
   } catch (const Failure& f) {

@@ -210,6 +210,11 @@ struct Scratch {
     if (trace) hipFree(trace);
     trace = nullptr;
     trace_n = trace_items = 0;
+    if (side) hipStreamDestroy(side);
+    if (fork) hipEventDestroy(fork);
+    if (join) hipEventDestroy(join);
+    side = nullptr;
+    fork = join = nullptr;
   }
   static void free_plan(PyrPlan* pl) {
     if (pl->dev) hipFree(pl->dev);
@@ -219,12 +224,48 @@ struct Scratch {
 };
 
 static std::mutex g_scratch_mu;
-static std::map<std::tuple<std::thread::id, hipStream_t, int>, Scratch*>* g_scratch =
-    new std::map<std::tuple<std::thread::id, hipStream_t, int>, Scratch*>();
+using ScratchKey = std::tuple<std::thread::id, hipStream_t, int>;
+static std::map<ScratchKey, Scratch*>* g_scratch = new std::map<ScratchKey, Scratch*>();
+static const std::thread::id g_main_thread = std::this_thread::get_id();
+
+// Frees every scratch set of the calling thread: at the exit of a thread
+// other than the one that loaded the library (the process's main thread
+// keeps its sets until the process ends, when the HIP runtime may already be
+// gone), so short-lived threads do not leave device memory and streams behind.
+static void release_thread_scratch() {
+  std::vector<Scratch*> mine;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    const std::thread::id me = std::this_thread::get_id();
+    for (auto it = g_scratch->begin(); it != g_scratch->end();) {
+      if (std::get<0>(it->first) == me) {
+        mine.push_back(it->second);
+        it = g_scratch->erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  for (Scratch* s : mine) {
+    // the stream may already be destroyed: wait for the whole device instead
+    (void)hipDeviceSynchronize();
+    s->release();
+    delete s;
+  }
+  (void)hipGetLastError();
+}
+struct ThreadScratchGuard {
+  bool armed = false;
+  ~ThreadScratchGuard() {
+    if (armed && std::this_thread::get_id() != g_main_thread) release_thread_scratch();
+  }
+};
+static thread_local ThreadScratchGuard t_scratch_guard;
 
 static Scratch& scratch_for(hipStream_t st) {
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
+  t_scratch_guard.armed = true;  // this thread's sets are freed at its exit
   std::lock_guard<std::mutex> lk(g_scratch_mu);
   auto key = std::make_tuple(std::this_thread::get_id(), st, dev);
   auto it = g_scratch->find(key);
@@ -1768,9 +1809,20 @@ long long dwt_gpu_scratch_bytes(void* stream) {
 
 int dwt_gpu_release_scratch(void* stream) {
   try {
-    Scratch& s = scratch_for((hipStream_t)stream);
-    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
-    s.release();
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    Scratch* s = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(g_scratch_mu);
+      auto it = g_scratch->find(std::make_tuple(std::this_thread::get_id(), (hipStream_t)stream, dev));
+      if (it == g_scratch->end()) return DWT_GPU_OK;
+      s = it->second;
+      g_scratch->erase(it);
+    }
+    const hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    s->release();
+    delete s;
+    if (e != hipSuccess) fail(DWT_GPU_EHIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
     return DWT_GPU_OK;
   } catch (const Failure& f) {
     return f.code;
