@@ -336,9 +336,9 @@ int dwt_gpu_tile_get(int forward, int width, int height, int wavelet, int *seg_r
 
 /* The small levels of a transform run as one single-workgroup launch that
  * keeps the whole small pyramid in LDS, from the first level j >= 1 whose
- * region is at most max_region_bytes (default 32 KiB, at most 48 KiB; 0
- * disables it).  An image of at most twice that is transformed completely by
- * that one launch. */
+ * region is at most max_region_bytes (default 64 KiB, at most 76 KiB; 0
+ * disables it) and whose sides are at most 512 samples.  An image of at most
+ * twice that is transformed completely by that one launch. */
 int dwt_gpu_tail_set(long long max_region_bytes);
 
 /* Optionally, three middle levels of the packed path run as one launch (each

@@ -1,6 +1,7 @@
 // Packed fast path: one 2D lifting level per launch, one HBM read of the
-// level input and one write of its four subbands; the single-workgroup
-// block kernels for small pyramids; host launchers and the tile-shape table.
+// level input and one write of its four subbands; copy kernels; host
+// launchers and the tile-shape table.  (The single-workgroup block kernels
+// for small pyramids are in block_kernels.hip.)
 // The tile bodies (work decomposition, lifting in registers) are in
 // tile_body.h.
 #include "tile_body.h"
@@ -52,111 +53,6 @@ inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ 
   if (strip >= nstrips) return;
   inv_tile<T, NS, U>(strip, s, S, W, H, GlobalLoad<T>{ll, llp},
                      GlobalDet<T>{const_cast<T*>(det), dp}, GlobalOut<T, VEC>{dst, dstp}, cf);
-}
-
-// ---- block kernels (one workgroup, a whole small pyramid in LDS) --------------
-// The small levels of a transform are latency-bound: every kernel boundary on
-// this multi-XCD GPU writes back and invalidates L2, and a level of a few KB
-// cannot fill the machine.  A block kernel does all levels from the first
-// small one on in one workgroup, with the same tile bodies as the level
-// kernels but every intermediate in LDS:
-//   * C (W0 x H0): the region's coefficients in Mallat layout;
-//   * P (ceil(W0/2) x ceil(H0/2)) and Q (ceil(W0/4) x ceil(H0/4)): the LL of
-//     odd and even levels.
-// Forward: level 0 reads the region from global memory, later levels from P /
-// Q; details go to C, the last LL to C; C is written back once.  Inverse: the
-// region is loaded into C once; each level reads its LL from C (coarsest) or
-// P / Q and its details from C, writing P / Q, and the finest level writes
-// the output to global memory.  The region is read completely before any of
-// it is written, so src may equal dst (an image this small is transformed by
-// one launch).  Sixteen wavefronts take the level's tiles in turn; a barrier
-// separates levels.
-constexpr int kBlockThreads = 1024;
-constexpr int kBlockRows = 8;  // segment rows of the tiles (single shot)
-
-template <typename T>
-struct BlockBufs {
-  Lds<T>* C;
-  Lds<T>* P;
-  Lds<T>* Q;
-  int W0, H0, pp, pq;
-  __device__ BlockBufs(Lds<T>* base, int W0_, int H0_) : W0(W0_), H0(H0_) {
-    pp = (W0 + 1) >> 1;
-    pq = (W0 + 3) >> 2;
-    C = base;
-    P = C + W0 * H0;
-    Q = P + pp * ((H0 + 1) >> 1);
-  }
-  // buffer and pitch of LL_j, 1 <= j
-  __device__ Lds<T>* ll(int j) const { return (j & 1) ? P : Q; }
-  __device__ int llp(int j) const { return (j & 1) ? pp : pq; }
-};
-
-template <typename T, int NS>
-__global__ void __launch_bounds__(kBlockThreads)
-block_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, long long dp, int W0,
-                 int H0, int nlev, Coeffs<T> cf) {
-  extern __shared__ __align__(16) unsigned char blk_smem[];
-  const BlockBufs<T> b((Lds<T>*)blk_smem, W0, H0);  // generic -> LDS address-space cast
-  constexpr int CORE = 2 * kWave - 2 * NS;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = kBlockThreads / kWave;
-  const LdsDet<T> det{b.C, W0};
-  for (int lev = 0; lev < nlev; ++lev) {
-    const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
-    const int nstrips = (W + CORE - 1) / CORE, ntiles = nstrips * ((H + kBlockRows - 1) / kBlockRows);
-    const bool last = lev == nlev - 1;
-    const LdsStore<T> ll{last ? b.C : b.ll(lev + 1), last ? W0 : b.llp(lev + 1)};
-    for (int t = wave; t < ntiles; t += nw) {
-      const int seg = t / nstrips, strip = t - seg * nstrips;
-      if (lev == 0)
-        fwd_tile<T, NS, -kBlockRows>(strip, seg, kBlockRows, W, H, GlobalRows<T, false>{src, sp, H}, ll,
-                                     det, cf);
-      else
-        fwd_tile<T, NS, -kBlockRows>(strip, seg, kBlockRows, W, H,
-                                     LdsRows<T>{b.ll(lev), b.llp(lev), H}, ll, det, cf);
-    }
-    __syncthreads();
-  }
-  Gbl<T>* g = gstore_ptr(dst);
-  for (int i = threadIdx.x; i < W0 * H0; i += kBlockThreads) {
-    const int y = i / W0, x = i - y * W0;
-    g[(long long)y * dp + x] = b.C[i];
-  }
-}
-
-template <typename T, int NS>
-__global__ void __launch_bounds__(kBlockThreads)
-block_inv_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, long long dp, int W0,
-                 int H0, int nlev, Coeffs<T> cf) {
-  extern __shared__ __align__(16) unsigned char blk_smem[];
-  const BlockBufs<T> b((Lds<T>*)blk_smem, W0, H0);  // generic -> LDS address-space cast
-  constexpr int CORE = 2 * kWave - 2 * NS;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), nw = kBlockThreads / kWave;
-  {
-    const Gbl<T>* g = gload_ptr(src);
-    for (int i = threadIdx.x; i < W0 * H0; i += kBlockThreads) {
-      const int y = i / W0, x = i - y * W0;
-      b.C[i] = g[(long long)y * sp + x];
-    }
-  }
-  __syncthreads();
-  const LdsDet<T> det{b.C, W0};
-  for (int lev = nlev - 1; lev >= 0; --lev) {
-    const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
-    const int nstrips = (W + CORE - 1) / CORE, ntiles = nstrips * ((H + kBlockRows - 1) / kBlockRows);
-    const bool coarsest = lev == nlev - 1;
-    const LdsLoad<T> ll{coarsest ? b.C : b.ll(lev + 1), coarsest ? W0 : b.llp(lev + 1)};
-    for (int t = wave; t < ntiles; t += nw) {
-      const int seg = t / nstrips, strip = t - seg * nstrips;
-      if (lev == 0)
-        inv_tile<T, NS, -kBlockRows>(strip, seg, kBlockRows, W, H, ll, det,
-                                     GlobalOut<T, false>{dst, dp}, cf);
-      else
-        inv_tile<T, NS, -kBlockRows>(strip, seg, kBlockRows, W, H, ll, det,
-                                     LdsOut<T>{b.ll(lev), b.llp(lev)}, cf);
-    }
-    __syncthreads();
-  }
 }
 
 template <typename T>
@@ -347,89 +243,6 @@ void launch_inv_level(const T* ll, long long llp, const T* det, long long dp, T*
                        dim3(ngroups * nseg), dim3(kWave * kWavesPerBlock), cf, st};
   with_variant<T>(cf.nsteps, aligned_pair<T>(dst, dstp), t, L);
 }
-
-// ---- block (tail) launchers --------------------------------------------------------
-// The block kernel takes over from the first level j >= 1 whose W x H region
-// is at most g_block_bytes, or from level 0 when the whole image is at most
-// twice that: one workgroup then replaces every launch of the transform
-// (measured on MI355X: 128 x 128 CDF 9/7 J=7 fwd+inv 65 us in one block vs
-// 95 us with a level-0 launch first; in a large image, regions above 32 KiB
-// are faster as one launch per level).  Its LDS holds C, P and Q: 1.3125
-// regions at most.  It reads its whole region before writing, so it works in
-// place.
-static long long g_block_bytes = 32 * 1024;
-constexpr int kBlockLdsMax = 128 * 1024;
-constexpr long long kBlockRegionMax = 48 * 1024;  // 2 x 48 KiB x 1.3125 < kBlockLdsMax
-
-void tail_config(long long max_region_bytes, int) {
-  g_block_bytes = max_region_bytes > kBlockRegionMax ? kBlockRegionMax : max_region_bytes;
-}
-
-static long long block_lds_bytes(long long W0, long long H0, int elem) {
-  return elem * (W0 * H0 + ((W0 + 1) / 2) * ((H0 + 1) / 2) + ((W0 + 3) / 4) * ((H0 + 3) / 4));
-}
-
-int tail_start(int W, int H, int L, int elem) {
-  if (g_block_bytes <= 0) return L;
-  for (int j = 0; j < L; ++j) {
-    const long long wj = ceil_div_pow2(W, j), hj = ceil_div_pow2(H, j);
-    const long long limit = j == 0 ? 2 * g_block_bytes : g_block_bytes;
-    if (wj * hj * elem <= limit && block_lds_bytes(wj, hj, elem) <= kBlockLdsMax) return j;
-  }
-  return L;
-}
-
-hipError_t configure_tail_kernels() {
-  const void* fns[] = {reinterpret_cast<const void*>(&block_fwd_kernel<float, 4>),
-                       reinterpret_cast<const void*>(&block_fwd_kernel<float, 2>),
-                       reinterpret_cast<const void*>(&block_fwd_kernel<double, 4>),
-                       reinterpret_cast<const void*>(&block_fwd_kernel<double, 2>),
-                       reinterpret_cast<const void*>(&block_inv_kernel<float, 4>),
-                       reinterpret_cast<const void*>(&block_inv_kernel<float, 2>),
-                       reinterpret_cast<const void*>(&block_inv_kernel<double, 4>),
-                       reinterpret_cast<const void*>(&block_inv_kernel<double, 2>)};
-  for (const void* f : fns) {
-    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kBlockLdsMax);
-    if (e != hipSuccess) {
-      (void)hipGetLastError();
-      return e;
-    }
-  }
-  return hipSuccess;
-}
-
-template <typename T>
-void launch_block_fwd(const T* src, long long sp, T* dst, long long dp, int W0, int H0, int nlev,
-                      const Coeffs<T>& cf, hipStream_t st) {
-  const size_t lds = (size_t)block_lds_bytes(W0, H0, (int)sizeof(T));
-  if (cf.nsteps == 4)
-    hipLaunchKernelGGL((block_fwd_kernel<T, 4>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst,
-                       dp, W0, H0, nlev, cf);
-  else
-    hipLaunchKernelGGL((block_fwd_kernel<T, 2>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst,
-                       dp, W0, H0, nlev, cf);
-}
-
-template <typename T>
-void launch_block_inv(const T* src, long long sp, T* dst, long long dp, int W0, int H0, int nlev,
-                      const Coeffs<T>& cf, hipStream_t st) {
-  const size_t lds = (size_t)block_lds_bytes(W0, H0, (int)sizeof(T));
-  if (cf.nsteps == 4)
-    hipLaunchKernelGGL((block_inv_kernel<T, 4>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst,
-                       dp, W0, H0, nlev, cf);
-  else
-    hipLaunchKernelGGL((block_inv_kernel<T, 2>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst,
-                       dp, W0, H0, nlev, cf);
-}
-
-template void launch_block_fwd<float>(const float*, long long, float*, long long, int, int, int,
-                                      const Coeffs<float>&, hipStream_t);
-template void launch_block_fwd<double>(const double*, long long, double*, long long, int, int, int,
-                                       const Coeffs<double>&, hipStream_t);
-template void launch_block_inv<float>(const float*, long long, float*, long long, int, int, int,
-                                      const Coeffs<float>&, hipStream_t);
-template void launch_block_inv<double>(const double*, long long, double*, long long, int, int, int,
-                                       const Coeffs<double>&, hipStream_t);
 
 template <typename T>
 void launch_copy_rect(const T* src, long long sp, T* dst, long long dp, int w, int h,

@@ -22,6 +22,8 @@ pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 FP32_TOL = 1e-5  # max-abs vs the FP64 oracle on the sinusoid+noise images
+TAIL_DEFAULT = 64 * 1024  # the library's default single-workgroup region
+TAIL_MAX = 76 * 1024      # the largest region dwt_gpu_tail_set accepts
 
 
 def say(msg):
@@ -186,7 +188,7 @@ def test_block_kernel_whole_images(prec):
     dt = np.float32 if prec == "s" else np.float64
     rng = np.random.default_rng(5)
     try:
-        for tb in (48 * 1024, 4 * 1024):
+        for tb in (TAIL_MAX, 4 * 1024):
             assert lib.dwt_gpu_tail_set(tb) == 0
             for (W, H) in BLOCK_SHAPES:
                 img = rng.standard_normal((H, W)).astype(dt)
@@ -202,7 +204,7 @@ def test_block_kernel_whole_images(prec):
                         np.testing.assert_array_equal(t.cpu().numpy(), o.inv2(ref, w, j),
                                                       err_msg="inverse")
     finally:
-        lib.dwt_gpu_tail_set(32 * 1024)
+        lib.dwt_gpu_tail_set(TAIL_DEFAULT)
 
 
 MID_SHAPES = [(1000, 700), (513, 511), (777, 1500), (257, 4000), (4000, 130), (96, 96), (300, 257)]
@@ -222,7 +224,7 @@ def test_mid_kernel_shapes(prec):
     rng = np.random.default_rng(9)
     try:
         assert lib.dwt_gpu_mid_set(1 << 40) == 0
-        for tb in (32 * 1024, 0):
+        for tb in (TAIL_DEFAULT, 0):
             assert lib.dwt_gpu_tail_set(tb) == 0
             for (W, H) in MID_SHAPES:
                 img = rng.standard_normal((H, W)).astype(dt)
@@ -239,7 +241,7 @@ def test_mid_kernel_shapes(prec):
                                                       err_msg="inverse")
     finally:
         lib.dwt_gpu_mid_set(0)
-        lib.dwt_gpu_tail_set(32 * 1024)
+        lib.dwt_gpu_tail_set(TAIL_DEFAULT)
 
 
 SPARSE_FRAMES = [((512, 512), (300, 257)), ((1024, 1024), (1000, 700)), ((64, 256), (33, 200)),
@@ -358,7 +360,7 @@ def test_pyramid_schedule_shapes(prec):
                     libdwt_amd.device_check(st)
                     np.testing.assert_array_equal(t.cpu().numpy(), want, err_msg=f"inverse {sched}")
     finally:
-        lib.dwt_gpu_tail_set(32 * 1024)
+        lib.dwt_gpu_tail_set(TAIL_DEFAULT)
         libdwt_amd.schedule_set(0)
 
 
