@@ -280,10 +280,17 @@ __device__ __forceinline__ void block_copy(Lds<T>* C, int pc, T* g, long long gp
     block_copy_rows<TO_LDS, T, 8>(C, pc, g, gp, W0, H0);
 }
 
+// Workgroup 0 transforms the region; any further workgroups move rows of a
+// level-0 detail copy riding on the launch (RiderCopy, kernels.h), on the
+// compute units the single transforming workgroup leaves idle.
 template <typename T, int NS>
 __global__ void __launch_bounds__(kBlockThreads)
 block_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, long long dp, int W0,
-                 int H0, int nlev, Coeffs<T> cf) {
+                 int H0, int nlev, Coeffs<T> cf, RiderCopy<T> rc) {
+  if (blockIdx.x > 0) {
+    rider_copy(rc, blockIdx.x - 1, gridDim.x - 1);
+    return;
+  }
   extern __shared__ __align__(16) unsigned char blk_smem[];
   Lds<T>* C = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
   const int pc = block_pitch(W0);
@@ -302,7 +309,11 @@ block_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, l
 template <typename T, int NS>
 __global__ void __launch_bounds__(kBlockThreads)
 block_inv_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, long long dp, int W0,
-                 int H0, int nlev, Coeffs<T> cf) {
+                 int H0, int nlev, Coeffs<T> cf, RiderCopy<T> rc) {
+  if (blockIdx.x > 0) {
+    rider_copy(rc, blockIdx.x - 1, gridDim.x - 1);
+    return;
+  }
   extern __shared__ __align__(16) unsigned char blk_smem[];
   Lds<T>* C = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
   const int pc = block_pitch(W0);
@@ -370,36 +381,47 @@ hipError_t configure_tail_kernels() {
 }
 
 template <typename T>
+static RiderCopy<T> block_rider(const RiderCopy<T>* r) {
+  if (r && r->r1 > r->r0 && r->blocks > 0) return *r;
+  RiderCopy<T> none{};
+  return none;
+}
+
+template <typename T>
 void launch_block_fwd(const T* src, long long sp, T* dst, long long dp, int W0, int H0, int nlev,
-                      const Coeffs<T>& cf, hipStream_t st) {
+                      const Coeffs<T>& cf, hipStream_t st, const RiderCopy<T>* rider) {
   const size_t lds = (size_t)block_lds_bytes(W0, H0, (int)sizeof(T));
+  const RiderCopy<T> rc = block_rider(rider);
+  const dim3 grid(1 + rc.blocks);
   if (cf.nsteps == 4)
-    hipLaunchKernelGGL((block_fwd_kernel<T, 4>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst,
-                       dp, W0, H0, nlev, cf);
+    hipLaunchKernelGGL((block_fwd_kernel<T, 4>), grid, dim3(kBlockThreads), lds, st, src, sp, dst, dp, W0,
+                       H0, nlev, cf, rc);
   else
-    hipLaunchKernelGGL((block_fwd_kernel<T, 2>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst,
-                       dp, W0, H0, nlev, cf);
+    hipLaunchKernelGGL((block_fwd_kernel<T, 2>), grid, dim3(kBlockThreads), lds, st, src, sp, dst, dp, W0,
+                       H0, nlev, cf, rc);
 }
 
 template <typename T>
 void launch_block_inv(const T* src, long long sp, T* dst, long long dp, int W0, int H0, int nlev,
-                      const Coeffs<T>& cf, hipStream_t st) {
+                      const Coeffs<T>& cf, hipStream_t st, const RiderCopy<T>* rider) {
   const size_t lds = (size_t)block_lds_bytes(W0, H0, (int)sizeof(T));
+  const RiderCopy<T> rc = block_rider(rider);
+  const dim3 grid(1 + rc.blocks);
   if (cf.nsteps == 4)
-    hipLaunchKernelGGL((block_inv_kernel<T, 4>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst,
-                       dp, W0, H0, nlev, cf);
+    hipLaunchKernelGGL((block_inv_kernel<T, 4>), grid, dim3(kBlockThreads), lds, st, src, sp, dst, dp, W0,
+                       H0, nlev, cf, rc);
   else
-    hipLaunchKernelGGL((block_inv_kernel<T, 2>), dim3(1), dim3(kBlockThreads), lds, st, src, sp, dst,
-                       dp, W0, H0, nlev, cf);
+    hipLaunchKernelGGL((block_inv_kernel<T, 2>), grid, dim3(kBlockThreads), lds, st, src, sp, dst, dp, W0,
+                       H0, nlev, cf, rc);
 }
 
 template void launch_block_fwd<float>(const float*, long long, float*, long long, int, int, int,
-                                      const Coeffs<float>&, hipStream_t);
+                                      const Coeffs<float>&, hipStream_t, const RiderCopy<float>*);
 template void launch_block_fwd<double>(const double*, long long, double*, long long, int, int, int,
-                                       const Coeffs<double>&, hipStream_t);
+                                       const Coeffs<double>&, hipStream_t, const RiderCopy<double>*);
 template void launch_block_inv<float>(const float*, long long, float*, long long, int, int, int,
-                                      const Coeffs<float>&, hipStream_t);
+                                      const Coeffs<float>&, hipStream_t, const RiderCopy<float>*);
 template void launch_block_inv<double>(const double*, long long, double*, long long, int, int, int,
-                                       const Coeffs<double>&, hipStream_t);
+                                       const Coeffs<double>&, hipStream_t, const RiderCopy<double>*);
 
 }  // namespace dwt

@@ -691,6 +691,77 @@ static int mid_start(int W, int H, int jt, int elem) {
   return -1;
 }
 
+// ---- the level-0 detail copy, carried by the small levels' launches -----------
+// Level 0 of the in-place forward writes its detail subbands to scratch D
+// (they overlap input that other workgroups have not read yet), and the last
+// level of the inverse reads them from D (its output overwrites them).  The
+// copy between D and the image rides on the launches of the later (forward) or
+// earlier (inverse) levels as extra workgroups: those levels touch only the
+// image's top-left LL_1 quadrant and the P buffers, disjoint from the copy,
+// and leave most of the GPU idle (the block kernel runs on one compute unit).
+// Weights: a level launch whose region is at most 8 MiB takes one share, the
+// block launch three; larger levels carry none.  With no carrier, the copy is a
+// launch of its own on the stream.
+template <typename T>
+static RiderCopy<T> detail_copy(const T* src, long long sp, T* dst, long long dp, int W, int H, int L) {
+  RiderCopy<T> rc{};
+  rc.src = src;
+  rc.sp = sp;
+  rc.dst = dst;
+  rc.dp = dp;
+  const int W1 = ceil_div_pow2(W, 1), H1 = ceil_div_pow2(H, 1);
+  rc.xa = L == 1 ? 0 : W1;  // with one level, LL_1 moves too
+  rc.wa = W - rc.xa;
+  rc.ha = H1;
+  rc.wb = W;
+  rc.hb = H - H1;
+  constexpr int per = 16 / (int)sizeof(T);
+  rc.vec = (reinterpret_cast<uintptr_t>(src) % 16) == 0 && (reinterpret_cast<uintptr_t>(dst) % 16) == 0 &&
+           sp % per == 0 && dp % per == 0 && rc.xa % per == 0;
+  return rc;
+}
+
+constexpr long long kCarrierBytes = 8ll << 20;
+constexpr int kBlockCarrierWeight = 3;
+
+// Assign rows [0, rows) of the copy to carriers by weight, in order; returns
+// false when no launch can carry any (all weights 0).
+static bool split_rows(int rows, const std::vector<int>& w, std::vector<int2>& out) {
+  long long tot = 0;
+  for (int x : w) tot += x;
+  out.assign(w.size(), make_int2(0, 0));
+  if (tot == 0) return false;
+  long long acc = 0;
+  int cur = 0;
+  for (size_t i = 0; i < w.size(); ++i) {
+    acc += w[i];
+    const int end = (int)(rows * acc / tot);
+    out[i] = make_int2(cur, end);
+    cur = end;
+  }
+  return true;
+}
+
+// The rows [r.x, r.y) of rc as carried by a launch whose workgroups have
+// `threads` threads (at most `cap` extra workgroups).
+template <typename T>
+static RiderCopy<T> rider_part(const RiderCopy<T>& rc, int2 r, int cap) {
+  RiderCopy<T> x = rc;
+  x.r0 = r.x;
+  x.r1 = r.y;
+  x.blocks = std::min(r.y - r.x, cap);
+  return x;
+}
+constexpr int kLevelRiderCap = 2048;  // 256-thread workgroups
+constexpr int kBlockRiderCap = 480;   // 1024-thread workgroups, two per compute unit
+
+template <typename T>
+static void copy_alone(const RiderCopy<T>& rc, hipStream_t st) {
+  LAUNCH(launch_copy_rect<T>(rc.src + rc.xa, rc.sp, rc.dst + rc.xa, rc.dp, rc.wa, rc.ha, st);
+         launch_copy_rect<T>(rc.src + (long long)rc.ha * rc.sp, rc.sp, rc.dst + (long long)rc.ha * rc.dp,
+                             rc.dp, rc.wb, rc.hb, st));
+}
+
 template <typename T>
 static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, Scratch& sc,
                      hipStream_t st) {
@@ -712,8 +783,19 @@ static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
   long long Pp[2] = {(long long)pad_pitch(W2), (long long)pad_pitch(W1)};
   if (L >= 2) P[1] = (T*)sc.get(1, sizeof(T) * (size_t)Pp[1] * H1);
   if (L >= 3) P[0] = (T*)sc.get(0, sizeof(T) * (size_t)Pp[0] * H2);
-  sc.ensure_side();
   const int jm = mid_start(W, H, jt, (int)sizeof(T));
+  // carriers of the detail copy: the level launches 1 .. jt-1 (not the fused
+  // middle levels) and the block launch
+  std::vector<int> weight;
+  for (int j = 1; j < jt; ++j) {
+    const bool mid = jm > 0 && j >= jm && j < jm + kMidLevels;
+    const long long bytes = (long long)ceil_div_pow2(W, j) * ceil_div_pow2(H, j) * (long long)sizeof(T);
+    weight.push_back(!mid && bytes <= kCarrierBytes ? 1 : 0);
+  }
+  if (jt < L) weight.push_back(kBlockCarrierWeight);
+  const RiderCopy<T> rc = detail_copy<T>(D, Dp, A, p, W, H, L);
+  std::vector<int2> rows;
+  const bool carried = split_rows(H, weight, rows);
   for (int j = 0; j < jt; ++j) {
     const int Wj = ceil_div_pow2(W, j), Hj = ceil_div_pow2(H, j);
     const T* src = j == 0 ? A : P[j % 2];
@@ -740,41 +822,36 @@ static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
     }
     T* det = j == 0 ? D : A;
     const long long dp = j == 0 ? Dp : p;
-    {
-      // level 0 reads A and writes only scratch, so a profiling pass may run
-      // it several times back to back inside one event pair (per-launch
-      // duration without the events' dispatch overhead)
-      const int reps = j == 0 ? prof_repeat() : 1;
-      ProfScope ps(0, j, st, reps);
-      for (int r = 0; r < reps; ++r)
-        LAUNCH(launch_fwd_level<T>(src, sp, Wj, Hj, ll, llp, det, dp, cf, st));
-    }
+    // level 0 reads A and writes only scratch, so a profiling pass may run
+    // it several times back to back inside one event pair (per-launch
+    // duration without the events' dispatch overhead)
+    const int reps = j == 0 ? prof_repeat() : 1;
+    ProfScope ps(0, j, st, reps);
     if (j == 0) {
-      // details of level 0 (and its LL when it is the only level) into A, on
-      // the side stream while the later levels run on st
-      HIPCHK(hipEventRecord(sc.fork, st));
-      HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
-      ProfScope ps(2, 0, sc.side);
-      LAUNCH(launch_copy_rect<T>(D + W1, Dp, A + W1, p, W - W1, H1, sc.side);
-             launch_copy_rect<T>(D + (long long)H1 * Dp, Dp, A + (long long)H1 * p, p, W, H - H1, sc.side);
-             if (L == 1) launch_copy_rect<T>(D, Dp, A, p, W1, H1, sc.side));
+      for (int r = 0; r < reps; ++r) LAUNCH(launch_fwd_level<T>(src, sp, Wj, Hj, ll, llp, det, dp, cf, st));
+    } else {
+      const RiderCopy<T> part = carried ? rider_part(rc, rows[j - 1], kLevelRiderCap) : RiderCopy<T>{};
+      LAUNCH(launch_fwd_level<T>(src, sp, Wj, Hj, ll, llp, det, dp, cf, st, &part));
     }
   }
   if (jt < L) {
     // levels jt .. L-1 in one block launch: LL_jt from P[jt % 2], the region's
     // coefficients into A
+    const RiderCopy<T> part = carried ? rider_part(rc, rows.back(), kBlockRiderCap) : RiderCopy<T>{};
     ProfScope ps(4, jt, st);
-    LAUNCH(launch_block_fwd<T>(P[jt % 2], Pp[jt % 2], A, p, ceil_div_pow2(W, jt),
-                               ceil_div_pow2(H, jt), L - jt, cf, st));
+    LAUNCH(launch_block_fwd<T>(P[jt % 2], Pp[jt % 2], A, p, ceil_div_pow2(W, jt), ceil_div_pow2(H, jt), L - jt,
+                               cf, st, &part));
   }
-  HIPCHK(hipEventRecord(sc.join, sc.side));
-  HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
+  if (!carried) {
+    ProfScope ps(2, 0, st);
+    copy_alone(rc, st);
+  }
 }
 
 // Inverse of fast_fwd: the block kernel does levels L .. jt+1 (writing LL_jt
 // to P[jt % 2]); level j >= 2 writes LL_{j-1} to P; level 1 writes the image
 // into A, so the level-0 details (and LL_1 when L == 1) are first copied out of
-// A into D on the side stream.
+// A into D, carried by the launches before that last one.
 template <typename T>
 static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, Scratch& sc,
                      hipStream_t st) {
@@ -796,22 +873,31 @@ static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
   long long Pp[2] = {(long long)pad_pitch(W2), (long long)pad_pitch(W1)};
   if (L >= 2) P[1] = (T*)sc.get(1, sizeof(T) * (size_t)Pp[1] * H1);
   if (L >= 3) P[0] = (T*)sc.get(0, sizeof(T) * (size_t)Pp[0] * H2);
-  sc.ensure_side();
-  HIPCHK(hipEventRecord(sc.fork, st));
-  HIPCHK(hipStreamWaitEvent(sc.side, sc.fork, 0));
-  {
-    ProfScope ps(2, 0, sc.side);
-    LAUNCH(launch_copy_rect<T>(A + W1, p, D + W1, Dp, W - W1, H1, sc.side);
-           launch_copy_rect<T>(A + (long long)H1 * p, p, D + (long long)H1 * Dp, Dp, W, H - H1, sc.side);
-           if (L == 1) launch_copy_rect<T>(A, p, D, Dp, W1, H1, sc.side));
-  }
-  HIPCHK(hipEventRecord(sc.join, sc.side));
-  if (jt < L) {
-    ProfScope ps(5, jt, st);
-    LAUNCH(launch_block_inv<T>(A, p, P[jt % 2], Pp[jt % 2], ceil_div_pow2(W, jt),
-                               ceil_div_pow2(H, jt), L - jt, cf, st));
-  }
   const int jm = mid_start(W, H, jt, (int)sizeof(T));
+  // carriers: the block launch, then the level launches jt .. 2 (each
+  // producing LL_{j-1}; not the fused middle levels)
+  std::vector<int> weight;
+  if (jt < L) weight.push_back(kBlockCarrierWeight);
+  for (int j = jt; j >= 2; --j) {
+    const bool mid = jm > 0 && j - 1 >= jm && j - 1 < jm + kMidLevels;
+    const long long bytes = (long long)ceil_div_pow2(W, j - 1) * ceil_div_pow2(H, j - 1) * (long long)sizeof(T);
+    weight.push_back(!mid && bytes <= kCarrierBytes ? 1 : 0);
+  }
+  const RiderCopy<T> rc = detail_copy<T>(A, p, D, Dp, W, H, L);
+  std::vector<int2> rows;
+  const bool carried = split_rows(H, weight, rows);
+  size_t carrier = 0;
+  if (!carried) {
+    ProfScope ps(2, 0, st);
+    copy_alone(rc, st);
+  }
+  if (jt < L) {
+    const RiderCopy<T> part = carried ? rider_part(rc, rows[carrier], kBlockRiderCap) : RiderCopy<T>{};
+    ++carrier;
+    ProfScope ps(5, jt, st);
+    LAUNCH(launch_block_inv<T>(A, p, P[jt % 2], Pp[jt % 2], ceil_div_pow2(W, jt), ceil_div_pow2(H, jt), L - jt,
+                               cf, st, &part));
+  }
   for (int j = jt; j >= 1; --j) {
     const int Wf = ceil_div_pow2(W, j - 1), Hf = ceil_div_pow2(H, j - 1);
     if (jm > 0 && j == jm + kMidLevels) {
@@ -822,6 +908,7 @@ static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
       ProfScope ps(11, jm, st);
       LAUNCH(launch_mid_inv<T>(llsrc, lp, A, p, P[jm % 2], Pp[jm % 2], ceil_div_pow2(W, jm),
                                ceil_div_pow2(H, jm), kMidLevels, cf, st));
+      carrier += kMidLevels;
       j = jm + 1;
       continue;
     }
@@ -838,9 +925,12 @@ static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
     const long long dp = j == 1 ? Dp : p;
     T* dst = j == 1 ? A : P[(j - 1) % 2];
     const long long dstp = j == 1 ? p : Pp[(j - 1) % 2];
-    if (j == 1) HIPCHK(hipStreamWaitEvent(st, sc.join, 0));
-    {
-      ProfScope ps(1, j - 1, st);
+    ProfScope ps(1, j - 1, st);
+    if (j >= 2) {
+      const RiderCopy<T> part = carried ? rider_part(rc, rows[carrier], kLevelRiderCap) : RiderCopy<T>{};
+      ++carrier;
+      LAUNCH(launch_inv_level<T>(ll, llp, det, dp, dst, dstp, Wf, Hf, cf, st, &part));
+    } else {
       LAUNCH(launch_inv_level<T>(ll, llp, det, dp, dst, dstp, Wf, Hf, cf, st));
     }
   }

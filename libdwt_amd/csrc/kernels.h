@@ -16,19 +16,42 @@ void launch_zero_pad(char* ptr, long long sx, long long sy, int fw, int fh, cons
 int line_lds_limit_bytes();
 hipError_t configure_line_kernels();
 
+// ---- level-0 detail copy carried by other launches ----------------------------
+// The in-place drivers move level 0's detail subbands between the image and
+// scratch (forward: scratch -> image after level 0; inverse: image -> scratch
+// before it).  Instead of a launch of its own, the copy rides on the launches
+// of the small levels, which leave most of the GPU idle: a launch carrying
+// rows [r0, r1) of the copy gets `blocks` extra workgroups that move them.
+// The rows form one list: row y < ha is columns [xa, xa + wa) of image row y,
+// row ha <= y < ha + hb is columns [0, wb) of image row y.  vec: 16-byte
+// access allowed (both pointers, pitches and xa suitably aligned).
+template <typename T>
+struct RiderCopy {
+  const T* src;
+  long long sp;
+  T* dst;
+  long long dp;
+  int xa, wa, ha, wb, hb;
+  int r0, r1;
+  int blocks;
+  int vec;
+};
+
 // ---- packed fast path (tile_kernels.hip) ------------------------------------
 // Forward level on a packed W x H region `src` (row pitch sp elements):
 // LL (ceil(W/2) x ceil(H/2)) goes to `ll` (pitch llp); HL, LH, HH go to `det`
 // (pitch dp) at their Mallat positions.  src must not overlap ll or det.
 template <typename T>
 void launch_fwd_level(const T* src, long long sp, int W, int H, T* ll, long long llp, T* det,
-                      long long dp, const Coeffs<T>& cf, hipStream_t st);
+                      long long dp, const Coeffs<T>& cf, hipStream_t st,
+                      const RiderCopy<T>* rider = nullptr);
 // Inverse level: LL from `ll` (pitch llp), HL/LH/HH from `det` (pitch dp) at
 // their Mallat positions of the W x H fine region; output to `dst` (pitch
 // dstp).  dst must not overlap ll or the details read from det.
 template <typename T>
 void launch_inv_level(const T* ll, long long llp, const T* det, long long dp, T* dst,
-                      long long dstp, int W, int H, const Coeffs<T>& cf, hipStream_t st);
+                      long long dstp, int W, int H, const Coeffs<T>& cf, hipStream_t st,
+                      const RiderCopy<T>* rider = nullptr);
 // Tile-shape table of the level kernels, by direction (1 forward, 0 inverse)
 // and height class ceil(log2(H)): S rows per segment, U prefetched row pairs
 // (S = 0 restores the built-in heuristic).  tile_get reports the shape a
@@ -43,10 +66,10 @@ int tile_get(int fwd, int H, int W, int nsteps, int* S, int* U);
 // L-level transform that the block kernel takes (L when none).
 template <typename T>
 void launch_block_fwd(const T* src, long long sp, T* dst, long long dp, int W0, int H0, int nlev,
-                      const Coeffs<T>& cf, hipStream_t st);
+                      const Coeffs<T>& cf, hipStream_t st, const RiderCopy<T>* rider = nullptr);
 template <typename T>
 void launch_block_inv(const T* src, long long sp, T* dst, long long dp, int W0, int H0, int nlev,
-                      const Coeffs<T>& cf, hipStream_t st);
+                      const Coeffs<T>& cf, hipStream_t st, const RiderCopy<T>* rider = nullptr);
 int tail_start(int W, int H, int L, int elem);
 void tail_config(long long max_region_bytes, int unused);
 hipError_t configure_tail_kernels();

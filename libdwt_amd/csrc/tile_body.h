@@ -32,6 +32,7 @@
 #include <type_traits>
 
 #include "dwt_common.h"
+#include "kernels.h"
 
 namespace dwt {
 
@@ -107,6 +108,29 @@ __device__ __forceinline__ void st_global(T* p, T v) {
 #else
   *gstore_ptr(p) = v;
 #endif
+}
+
+// Rows of a RiderCopy (kernels.h) moved by one of its `nwg` extra
+// workgroups, whose index among them is k: rows r0 + k, r0 + k + nwg, ...
+typedef float Vec16f __attribute__((ext_vector_type(4)));
+typedef double Vec16d __attribute__((ext_vector_type(2)));
+template <typename T>
+__device__ __forceinline__ void rider_copy(const RiderCopy<T>& rc, int k, int nwg) {
+  using V = typename std::conditional<sizeof(T) == 4, Vec16f, Vec16d>::type;
+  constexpr int PER = 16 / sizeof(T);  // elements of one 16-byte vector
+  for (int y = rc.r0 + k; y < rc.r1; y += nwg) {
+    const int x0 = y < rc.ha ? rc.xa : 0, w = y < rc.ha ? rc.wa : rc.wb;
+    const Gbl<T>* s = gload_ptr(rc.src) + (long long)y * rc.sp + x0;
+    Gbl<T>* d = gstore_ptr(rc.dst) + (long long)y * rc.dp + x0;
+    if (rc.vec) {
+      const int nv = w / PER;
+      for (int i = threadIdx.x; i < nv; i += blockDim.x)
+        reinterpret_cast<Gbl<V>*>(d)[i] = reinterpret_cast<const Gbl<V>*>(s)[i];
+      for (int i = nv * PER + threadIdx.x; i < w; i += blockDim.x) d[i] = s[i];
+    } else {
+      for (int i = threadIdx.x; i < w; i += blockDim.x) d[i] = s[i];
+    }
+  }
 }
 
 // Two-element vectors as clang ext vectors (HIP's vector classes have

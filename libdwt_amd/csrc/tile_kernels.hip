@@ -27,13 +27,19 @@ __device__ inline int xcd_band_remap(int b, int nb, int ngroups) {
   return (x + 8 * (idx / C)) * C + idx % C;
 }
 
+// Workgroups past the level's tiles (blockIdx.x >= tile_blocks) move rows of
+// a level-0 detail copy riding on this launch (RiderCopy, kernels.h).
 template <typename T, int NS, bool VEC, int U>
 __global__ void __launch_bounds__(256)
 fwd_level_kernel(const T* __restrict__ src, long long sp, int W, int H, T* __restrict__ ll,
                  long long llp, T* __restrict__ det, long long dp, int S, int nstrips, int nseg,
-                 Coeffs<T> cf) {
+                 Coeffs<T> cf, int tile_blocks, RiderCopy<T> rc) {
+  if ((int)blockIdx.x >= tile_blocks) {
+    rider_copy(rc, blockIdx.x - tile_blocks, gridDim.x - tile_blocks);
+    return;
+  }
   const int ngroups = (nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int b = xcd_band_remap(blockIdx.x, gridDim.x, ngroups);
+  const int b = xcd_band_remap(blockIdx.x, tile_blocks, ngroups);
   const int s = b / ngroups, g = b - s * ngroups;
   const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
@@ -45,9 +51,13 @@ template <typename T, int NS, bool VEC, int U>
 __global__ void __launch_bounds__(256)
 inv_level_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ det, long long dp,
                  T* __restrict__ dst, long long dstp, int W, int H, int S, int nstrips, int nseg,
-                 Coeffs<T> cf) {
+                 Coeffs<T> cf, int tile_blocks, RiderCopy<T> rc) {
+  if ((int)blockIdx.x >= tile_blocks) {
+    rider_copy(rc, blockIdx.x - tile_blocks, gridDim.x - tile_blocks);
+    return;
+  }
   const int ngroups = (nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
-  const int b = xcd_remap(blockIdx.x, gridDim.x);  // bands measured slower for the inverse
+  const int b = xcd_remap(blockIdx.x, tile_blocks);  // bands measured slower for the inverse
   const int s = b / ngroups, g = b - s * ngroups;
   const int strip = g * kWavesPerBlock + (threadIdx.x >> 6);
   if (strip >= nstrips) return;
@@ -199,48 +209,60 @@ static void with_variant(int nsteps, bool vec, const TileShape& t, const Launch&
 template <typename T>
 struct FwdLaunch {
   const T* src; long long sp; int W, H; T* ll; long long llp; T* det; long long dp;
-  int S, nstrips, nseg; dim3 grid, block; Coeffs<T> cf; hipStream_t st;
+  int S, nstrips, nseg; dim3 grid, block; Coeffs<T> cf; hipStream_t st; int tiles; RiderCopy<T> rc;
   template <typename TT, int NS, bool VEC, int U>
   void go() const {
     hipLaunchKernelGGL((fwd_level_kernel<TT, NS, VEC, U>), grid, block, 0, st, src, sp, W, H, ll,
-                       llp, det, dp, S, nstrips, nseg, cf);
+                       llp, det, dp, S, nstrips, nseg, cf, tiles, rc);
   }
 };
 
 template <typename T>
 struct InvLaunch {
   const T* ll; long long llp; const T* det; long long dp; T* dst; long long dstp; int W, H;
-  int S, nstrips, nseg; dim3 grid, block; Coeffs<T> cf; hipStream_t st;
+  int S, nstrips, nseg; dim3 grid, block; Coeffs<T> cf; hipStream_t st; int tiles; RiderCopy<T> rc;
   template <typename TT, int NS, bool VEC, int U>
   void go() const {
     hipLaunchKernelGGL((inv_level_kernel<TT, NS, VEC, U>), grid, block, 0, st, ll, llp, det, dp,
-                       dst, dstp, W, H, S, nstrips, nseg, cf);
+                       dst, dstp, W, H, S, nstrips, nseg, cf, tiles, rc);
   }
 };
 
 template <typename T>
+static RiderCopy<T> rider_or_none(const RiderCopy<T>* r) {
+  if (r && r->r1 > r->r0 && r->blocks > 0) return *r;
+  RiderCopy<T> none{};
+  return none;
+}
+
+template <typename T>
 void launch_fwd_level(const T* src, long long sp, int W, int H, T* ll, long long llp, T* det,
-                      long long dp, const Coeffs<T>& cf, hipStream_t st) {
+                      long long dp, const Coeffs<T>& cf, hipStream_t st, const RiderCopy<T>* rider) {
   const int core = 2 * kWave - 2 * cf.nsteps;
   const int nstrips = (W + core - 1) / core;
   const TileShape t = pick_tile(H, nstrips, true);
   const int nseg = (H + t.S - 1) / t.S;
   const int ngroups = (nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
+  const RiderCopy<T> rc = rider_or_none(rider);
+  const int tiles = ngroups * nseg;
   const FwdLaunch<T> L{src, sp, W, H, ll, llp, det, dp, t.S, nstrips, nseg,
-                       dim3(ngroups * nseg), dim3(kWave * kWavesPerBlock), cf, st};
+                       dim3(tiles + rc.blocks), dim3(kWave * kWavesPerBlock), cf, st, tiles, rc};
   with_variant<T>(cf.nsteps, aligned_pair<T>(src, sp), t, L);
 }
 
 template <typename T>
 void launch_inv_level(const T* ll, long long llp, const T* det, long long dp, T* dst,
-                      long long dstp, int W, int H, const Coeffs<T>& cf, hipStream_t st) {
+                      long long dstp, int W, int H, const Coeffs<T>& cf, hipStream_t st,
+                      const RiderCopy<T>* rider) {
   const int core = 2 * kWave - 2 * cf.nsteps;
   const int nstrips = (W + core - 1) / core;
   const TileShape t = pick_tile(H, nstrips, false);
   const int nseg = (H + t.S - 1) / t.S;
   const int ngroups = (nstrips + kWavesPerBlock - 1) / kWavesPerBlock;
+  const RiderCopy<T> rc = rider_or_none(rider);
+  const int tiles = ngroups * nseg;
   const InvLaunch<T> L{ll, llp, det, dp, dst, dstp, W, H, t.S, nstrips, nseg,
-                       dim3(ngroups * nseg), dim3(kWave * kWavesPerBlock), cf, st};
+                       dim3(tiles + rc.blocks), dim3(kWave * kWavesPerBlock), cf, st, tiles, rc};
   with_variant<T>(cf.nsteps, aligned_pair<T>(dst, dstp), t, L);
 }
 
@@ -294,14 +316,17 @@ template void launch_channels<float>(bool, float*, long long, long long, long lo
 template void launch_channels<double>(bool, double*, long long, long long, long long, int, int, int,
                                       double*, long long, long long, hipStream_t);
 template void launch_fwd_level<float>(const float*, long long, int, int, float*, long long,
-                                      float*, long long, const Coeffs<float>&, hipStream_t);
+                                      float*, long long, const Coeffs<float>&, hipStream_t,
+                                      const RiderCopy<float>*);
 template void launch_fwd_level<double>(const double*, long long, int, int, double*, long long,
-                                       double*, long long, const Coeffs<double>&, hipStream_t);
+                                       double*, long long, const Coeffs<double>&, hipStream_t,
+                                       const RiderCopy<double>*);
 template void launch_inv_level<float>(const float*, long long, const float*, long long, float*,
-                                      long long, int, int, const Coeffs<float>&, hipStream_t);
+                                      long long, int, int, const Coeffs<float>&, hipStream_t,
+                                      const RiderCopy<float>*);
 template void launch_inv_level<double>(const double*, long long, const double*, long long,
                                        double*, long long, int, int, const Coeffs<double>&,
-                                       hipStream_t);
+                                       hipStream_t, const RiderCopy<double>*);
 template void launch_copy_rect<float>(const float*, long long, float*, long long, int, int,
                                       hipStream_t);
 template void launch_copy_rect<double>(const double*, long long, double*, long long, int, int,
