@@ -251,6 +251,27 @@ int dwt_gpu_2d_async(int wavelet, int direction, int precision, void *ptr, int s
                      int size_i_big_y, int *j, int decompose_one, int zero_padding,
                      void *stream);
 
+/* Allocation-free variant of dwt_gpu_2d_async.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory -- code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * The call allocates nothing (no hipMalloc / hipFree, no streams or events, no
+ * library routine that allocates); its scratch is a static device array of
+ * DWT_GPU_NOALLOC_SCRATCH_BYTES that every GPU the library runs on holds until
+ * the process ends.  Frames: packed FP32 in device memory (ptr and stride_x
+ * multiples of 4 bytes, element stride 4 bytes, outer frame = inner frame,
+ * decompose_one = 0, zero_padding irrelevant).  *j as dwt_gpu_2d_async (read
+ * for the inverse, clamped and written for the forward).  An image whose
+ * scratch exceeds the array -- W x H x 4 x (1 + 1/4 + 1/16) bytes with padded
+ * pitches, about 12.7 Mpixel; an image the single-workgroup kernel takes
+ * whole (at most 128 KiB, sides at most 512) needs none -- is rejected with
+ * DWT_GPU_EINVAL before anything is enqueued.  Asynchronous on
+ * `stream`.  Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU (other streams, other host threads) race; calls on one
+ * stream are ordered. */
+#define DWT_GPU_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int dwt_gpu_2d_noalloc(int wavelet, int direction, void *ptr, int stride_x, int size_x, int size_y,
+                       int *j, void *stream);
+
 /* Every channel of a multi-channel image in one call.  Channel c is the
  * size_x x size_y frame at ptr + c * channel_stride (bytes), rows stride_x
  * and pixels stride_y bytes apart, as the drivers take them (outer frame =

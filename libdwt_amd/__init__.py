@@ -99,6 +99,8 @@ def lib() -> C.CDLL:
         L.dwt_gpu_version.restype = C.c_char_p
         L.dwt_gpu_2d_async.argtypes = [i, i, i, vp, i, i, i, i, i, i, C.POINTER(i), i, i, vp]
         L.dwt_gpu_2d_async.restype = i
+        L.dwt_gpu_2d_noalloc.argtypes = [i, i, vp, i, i, i, C.POINTER(i), vp]
+        L.dwt_gpu_2d_noalloc.restype = i
         L.dwt_gpu_2d_channels.argtypes = [i, i, i, vp, i, i, i, i, i, i, C.POINTER(i), i, i, vp]
         L.dwt_gpu_2d_channels.restype = i
         L.dwt_gpu_profile_enable.argtypes = [i]

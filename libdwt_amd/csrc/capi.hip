@@ -762,27 +762,61 @@ static void copy_alone(const RiderCopy<T>& rc, hipStream_t st) {
                              rc.dp, rc.wb, rc.hb, st));
 }
 
+// Scratch of the per-level schedule: D (level-0 details, W x H) and the
+// ping-pong LL buffers P[0] (LL of even levels >= 2, W_2 x H_2) and P[1] (LL of
+// odd levels, W_1 x H_1), pitches padded to 64 elements.  With base == null
+// only the byte count is computed (each buffer 256-byte aligned).
 template <typename T>
-static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, Scratch& sc,
-                     hipStream_t st) {
+struct FastBufs {
+  T* D = nullptr;
+  T* P[2] = {nullptr, nullptr};
+  long long Dp = 0, Pp[2] = {0, 0};
+};
+template <typename T>
+static size_t fast_bufs_layout(int W, int H, int L, char* base, FastBufs<T>* b) {
+  const int W1 = ceil_div_pow2(W, 1), H1 = ceil_div_pow2(H, 1);
+  const int W2 = ceil_div_pow2(W, 2), H2 = ceil_div_pow2(H, 2);
+  b->Dp = (long long)pad_pitch(W);
+  b->Pp[0] = (long long)pad_pitch(W2);
+  b->Pp[1] = (long long)pad_pitch(W1);
+  const size_t sz[3] = {sizeof(T) * (size_t)b->Dp * H, L >= 2 ? sizeof(T) * (size_t)b->Pp[1] * H1 : 0,
+                        L >= 3 ? sizeof(T) * (size_t)b->Pp[0] * H2 : 0};
+  size_t off = 0, at[3];
+  for (int i = 0; i < 3; ++i) {
+    at[i] = off;
+    off += (sz[i] + 255) & ~size_t(255);
+  }
+  if (base) {
+    b->D = reinterpret_cast<T*>(base + at[0]);
+    b->P[1] = sz[1] ? reinterpret_cast<T*>(base + at[1]) : nullptr;
+    b->P[0] = sz[2] ? reinterpret_cast<T*>(base + at[2]) : nullptr;
+  }
+  return off;
+}
+template <typename T>
+static FastBufs<T> fast_bufs(Scratch& sc, int W, int H, int L) {
+  FastBufs<T> b;
+  fast_bufs_layout<T>(W, H, L, nullptr, &b);
+  b.D = (T*)sc.get(2, sizeof(T) * (size_t)b.Dp * H);
+  if (L >= 2) b.P[1] = (T*)sc.get(1, sizeof(T) * (size_t)b.Pp[1] * ceil_div_pow2(H, 1));
+  if (L >= 3) b.P[0] = (T*)sc.get(0, sizeof(T) * (size_t)b.Pp[0] * ceil_div_pow2(H, 2));
+  return b;
+}
+
+// Forward levels [0, L) of a packed image on the given scratch buffers (unused
+// when the block kernel takes the whole image).  Launches kernels only.
+template <typename T>
+static void fast_fwd_on(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, const FastBufs<T>& fb,
+                        hipStream_t st) {
   const int jt = tail_start(W, H, L, (int)sizeof(T));
   if (jt == 0) {
     ProfScope ps(4, 0, st);
     LAUNCH(launch_block_fwd<T>(A, p, A, p, W, H, L, cf, st));
     return;
   }
-  if (g_schedule == 1) {
-    fast_fwd_pyr<T>(A, p, W, H, L, cf, sc, st);
-    return;
-  }
-  const int W1 = ceil_div_pow2(W, 1), H1 = ceil_div_pow2(H, 1);
-  const int W2 = ceil_div_pow2(W, 2), H2 = ceil_div_pow2(H, 2);
-  const long long Dp = (long long)pad_pitch(W);
-  T* D = (T*)sc.get(2, sizeof(T) * (size_t)Dp * H);
-  T* P[2] = {nullptr, nullptr};
-  long long Pp[2] = {(long long)pad_pitch(W2), (long long)pad_pitch(W1)};
-  if (L >= 2) P[1] = (T*)sc.get(1, sizeof(T) * (size_t)Pp[1] * H1);
-  if (L >= 3) P[0] = (T*)sc.get(0, sizeof(T) * (size_t)Pp[0] * H2);
+  T* const D = fb.D;
+  T* const P[2] = {fb.P[0], fb.P[1]};
+  const long long Dp = fb.Dp, Pp[2] = {fb.Pp[0], fb.Pp[1]};
   const int jm = mid_start(W, H, jt, (int)sizeof(T));
   // carriers of the detail copy: the level launches 1 .. jt-1 (not the fused
   // middle levels) and the block launch
@@ -848,31 +882,33 @@ static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
   }
 }
 
+template <typename T>
+static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, Scratch& sc,
+                     hipStream_t st) {
+  if (tail_start(W, H, L, (int)sizeof(T)) > 0 && g_schedule == 1) {
+    fast_fwd_pyr<T>(A, p, W, H, L, cf, sc, st);
+    return;
+  }
+  const FastBufs<T> fb = tail_start(W, H, L, (int)sizeof(T)) > 0 ? fast_bufs<T>(sc, W, H, L) : FastBufs<T>{};
+  fast_fwd_on<T>(A, p, W, H, L, cf, fb, st);
+}
+
 // Inverse of fast_fwd: the block kernel does levels L .. jt+1 (writing LL_jt
 // to P[jt % 2]); level j >= 2 writes LL_{j-1} to P; level 1 writes the image
 // into A, so the level-0 details (and LL_1 when L == 1) are first copied out of
 // A into D, carried by the launches before that last one.
 template <typename T>
-static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, Scratch& sc,
-                     hipStream_t st) {
+static void fast_inv_on(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, const FastBufs<T>& fb,
+                        hipStream_t st) {
   const int jt = tail_start(W, H, L, (int)sizeof(T));
   if (jt == 0) {
     ProfScope ps(5, 0, st);
     LAUNCH(launch_block_inv<T>(A, p, A, p, W, H, L, cf, st));
     return;
   }
-  if (g_schedule == 1) {
-    fast_inv_pyr<T>(A, p, W, H, L, cf, sc, st);
-    return;
-  }
-  const int W1 = ceil_div_pow2(W, 1), H1 = ceil_div_pow2(H, 1);
-  const int W2 = ceil_div_pow2(W, 2), H2 = ceil_div_pow2(H, 2);
-  const long long Dp = (long long)pad_pitch(W);
-  T* D = (T*)sc.get(2, sizeof(T) * (size_t)Dp * H);
-  T* P[2] = {nullptr, nullptr};
-  long long Pp[2] = {(long long)pad_pitch(W2), (long long)pad_pitch(W1)};
-  if (L >= 2) P[1] = (T*)sc.get(1, sizeof(T) * (size_t)Pp[1] * H1);
-  if (L >= 3) P[0] = (T*)sc.get(0, sizeof(T) * (size_t)Pp[0] * H2);
+  T* const D = fb.D;
+  T* const P[2] = {fb.P[0], fb.P[1]};
+  const long long Dp = fb.Dp, Pp[2] = {fb.Pp[0], fb.Pp[1]};
   const int jm = mid_start(W, H, jt, (int)sizeof(T));
   // carriers: the block launch, then the level launches jt .. 2 (each
   // producing LL_{j-1}; not the fused middle levels)
@@ -934,6 +970,17 @@ static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
       LAUNCH(launch_inv_level<T>(ll, llp, det, dp, dst, dstp, Wf, Hf, cf, st));
     }
   }
+}
+
+template <typename T>
+static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, Scratch& sc,
+                     hipStream_t st) {
+  if (tail_start(W, H, L, (int)sizeof(T)) > 0 && g_schedule == 1) {
+    fast_inv_pyr<T>(A, p, W, H, L, cf, sc, st);
+    return;
+  }
+  const FastBufs<T> fb = tail_start(W, H, L, (int)sizeof(T)) > 0 ? fast_bufs<T>(sc, W, H, L) : FastBufs<T>{};
+  fast_inv_on<T>(A, p, W, H, L, cf, fb, st);
 }
 
 // ---- 2D driver on device-accessible memory -------------------------------------
@@ -1476,6 +1523,72 @@ static void run_lines(bool fwd, char* ptr, long long sig, long long smp, int N, 
   std::abort();
 }
 
+// ---- allocation-free entry point: scratch in a static device array ------------
+// dwt_gpu_2d_noalloc runs the packed fast path on scratch carved out of this
+// file-scope array instead of allocated memory.  Every GPU the library runs on
+// holds its own copy of it until the process ends; on the host the array's
+// name is only a stand-in, so its device address is taken once per device with
+// hipGetSymbolAddress and only that address is used.
+constexpr size_t kNoAllocScratchBytes = (size_t)DWT_GPU_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_noalloc_mu;
+static std::map<int, char*>* g_noalloc_ptr = new std::map<int, char*>();
+
+static char* noalloc_scratch() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  auto it = g_noalloc_ptr->find(dev);
+  if (it != g_noalloc_ptr->end()) return it->second;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    (void)hipGetLastError();
+    fail(DWT_GPU_EHIP, "hipGetSymbolAddress of the static scratch: %s", hipGetErrorString(e));
+  }
+  (*g_noalloc_ptr)[dev] = static_cast<char*>(ptr);
+  return static_cast<char*>(ptr);
+}
+
+// The packed FP32 transform of dwt_gpu_2d_async on the static scratch; levels
+// as the forward driver clamps them (decompose_one = 0, so every level has both
+// sides >= 2 and takes the packed path).
+static void run2d_noalloc(bool fwd, Wavelet w, void* ptr, int sx, int W, int H, int* jp, hipStream_t st) {
+  init_once();
+  if (!ptr) fail(DWT_GPU_EINVAL, "null image pointer");
+  if (!jp) fail(DWT_GPU_EINVAL, "null j");
+  if (W < 0 || H < 0) fail(DWT_GPU_EINVAL, "negative frame size");
+  if (sx <= 0 || sx % (int)sizeof(float) != 0 || (long long)sx < (long long)W * (long long)sizeof(float))
+    fail(DWT_GPU_EINVAL, "stride_x must be a positive multiple of 4 bytes and hold a row");
+  if (reinterpret_cast<uintptr_t>(ptr) % sizeof(float) != 0) fail(DWT_GPU_EINVAL, "image pointer not 4-byte aligned");
+  void* dptr = device_address(ptr);
+  if (!dptr || dptr != ptr) fail(DWT_GPU_EMEMORY, "image pointer is not device memory");
+  const int jl = level_limit(W, H, 0);
+  int J = *jp;
+  if (fwd) {
+    if (J < 0 || J > jl) J = jl;  // src/libdwt.c:6326-6329
+    *jp = J;
+  } else if (J < 0 || J > jl) {
+    J = jl;  // src/libdwt.c:6678-6681
+  }
+  if (J == 0 || W == 0 || H == 0) return;
+  FastBufs<float> fb;
+  if (tail_start(W, H, J, (int)sizeof(float)) > 0) {
+    const size_t need = fast_bufs_layout<float>(W, H, J, nullptr, &fb);
+    if (need > kNoAllocScratchBytes)
+      fail(DWT_GPU_EINVAL, "%d x %d needs %zu bytes of scratch; the allocation-free entry point holds %zu", W, H,
+           need, kNoAllocScratchBytes);
+    fast_bufs_layout<float>(W, H, J, noalloc_scratch(), &fb);
+  }
+  const Coeffs<float> cf = make_coeffs<float>(w, fwd);
+  float* A = static_cast<float*>(ptr);
+  const long long p = sx / (long long)sizeof(float);
+  if (fwd)
+    fast_fwd_on<float>(A, p, W, H, J, cf, fb, st);
+  else
+    fast_inv_on<float>(A, p, W, H, J, cf, fb, st);
+}
+
 }  // namespace dwt
 
 using namespace dwt;
@@ -1611,6 +1724,20 @@ int dwt_gpu_2d_async(int wavelet, int direction, int precision, void* ptr, int s
     else
       run2d<double>(fwd, dptr, stride_x, stride_y, size_o_big_x, size_o_big_y, size_i_big_x,
                     size_i_big_y, j, decompose_one, zero_padding, w, st, false);
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
+}
+
+int dwt_gpu_2d_noalloc(int wavelet, int direction, void* ptr, int stride_x, int size_x, int size_y, int* j,
+                       void* stream) {
+  try {
+    g_err.clear();
+    if (wavelet != DWT_GPU_CDF97 && wavelet != DWT_GPU_CDF53) fail(DWT_GPU_EINVAL, "bad wavelet");
+    if (direction != DWT_GPU_FORWARD && direction != DWT_GPU_INVERSE) fail(DWT_GPU_EINVAL, "bad direction");
+    run2d_noalloc(direction == DWT_GPU_FORWARD, wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53, ptr, stride_x, size_x,
+                  size_y, j, (hipStream_t)stream);
     return DWT_GPU_OK;
   } catch (const Failure& f) {
     return f.code;
