@@ -24,6 +24,7 @@
 #ifndef LIBDWT_H
 #define LIBDWT_H
 
+#include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
 
@@ -195,6 +196,26 @@ void dwt_util_perf_cdf97_2_d(int stride_x, int stride_y, int size_o_big_x, int s
                              int zero_padding, int M, int N, int clock_type, double *fwd_secs,
                              double *inv_secs);
 int dwt_util_pow2_ceil_log2(int x);
+/* More host utilities (reference: src/libdwt.h:606, 744, 803, 950, 961-1001,
+ * 1010-1036): ceil(log2 x); whether a timer type is available (0) or not
+ * (-1); the worker count (1: no ASVP workers); an active wait of ms
+ * milliseconds; float vectors (allocate 8-byte aligned, even size; fill with
+ * 0, 1, 2, ...; zero; copy; compare -- the last four return non-zero when a
+ * value is not normal-or-zero or differs by more than 1e-4); printf
+ * replacements. */
+int dwt_util_ceil_log2(int x);
+int dwt_util_clock_available(int type);
+int dwt_util_get_max_workers(void);
+void dwt_util_wait(int ms);
+float *dwt_util_allocate_vec_s(int size);
+int dwt_util_generate_vec_s(float *addr, int size);
+int dwt_util_zero_vec_s(float *addr, int size);
+int dwt_util_copy_vec_s(const float *src, float *dst, int size);
+int dwt_util_cmp_vec_s(const float *a, const float *b, int size);
+int dwt_util_vfprintf(FILE *stream, const char *format, va_list ap);
+int dwt_util_vprintf(const char *format, va_list ap);
+int dwt_util_fprintf(FILE *stream, const char *format, ...);
+int dwt_util_printf(const char *format, ...);
 void dwt_util_get_sizes_s(enum dwt_array array_type, int size_x, int size_y, int opt_stride,
                           int *stride_x, int *stride_y, int *size_o_big_x, int *size_o_big_y,
                           int *size_i_big_x, int *size_i_big_y);
@@ -487,6 +508,21 @@ int dwt_gpu_server_drain(void *server);
 int dwt_gpu_server_stats(void *server, long long *batches, long long *requests);
 /* Waits for the requests in flight, then frees the server. */
 int dwt_gpu_server_destroy(void *server);
+
+/* Test support (no counterpart in the reference): the staged in-place level 0
+ * of dwt_gpu_1d_batch for signals stored as matrix columns.  dwt_gpu_col_plan
+ * returns 1 when level 0 of an n-row level runs in place as segment ranges
+ * [stages[k], stages[k+1]) (k < 3; forward launched from the highest range
+ * down, inverse from the lowest up) plus the top segments [stages[3],
+ * segments) whose H rows from *top_h_row on pass through scratch, 0 when it
+ * runs through scratch entirely.  dwt_gpu_col_segment_rows lists the rows one
+ * segment reads and writes, computed by the column kernels' own index
+ * arithmetic: forward, input rows read and H rows written (at ceil(n/2) + m);
+ * inverse, H rows read and output rows written. */
+int dwt_gpu_col_plan(int forward, int n, int nsteps, int *stages, int *top_h_row, int *segments,
+                     int *rows_per_segment);
+int dwt_gpu_col_segment_rows(int forward, int n, int nsteps, int seg, int *reads, int max_reads,
+                             int *nreads, int *writes, int max_writes, int *nwrites);
 
 /* Bytes of device scratch the calling thread currently holds for `stream`.
  * Scratch is kept per (host thread, stream, device) and reused by later calls;

@@ -2002,6 +2002,43 @@ int dwt_gpu_1d_batch(int wavelet, int direction, int precision, void* ptr, long 
   }
 }
 
+int dwt_gpu_col_plan(int forward, int n, int nsteps, int* stages, int* top_h_row, int* segments,
+                     int* rows_per_segment) {
+  if (!stages || !top_h_row || !segments || !rows_per_segment || n < 0 || (nsteps != 2 && nsteps != 4)) {
+    g_err = "dwt_gpu_col_plan: bad argument";
+    return DWT_GPU_EINVAL;
+  }
+  *segments = col_segments(n);
+  *rows_per_segment = col_segment_rows();
+  int s[kColStages + 1] = {};
+  const bool staged = col_stages(forward != 0, n, nsteps, s);
+  for (int k = 0; k <= kColStages; ++k) stages[k] = staged ? s[k] : 0;
+  *top_h_row = staged ? (forward ? s[kColStages] * col_segment_rows() / 2
+                                 : s[kColStages] * col_segment_rows() / 2 - nsteps / 2)
+                      : 0;
+  return staged ? 1 : 0;
+}
+
+int dwt_gpu_col_segment_rows(int forward, int n, int nsteps, int seg, int* reads, int max_reads, int* nreads,
+                             int* writes, int max_writes, int* nwrites) {
+  if (!reads || !writes || !nreads || !nwrites || n < 2 || seg < 0 || seg >= col_segments(n) ||
+      (nsteps != 2 && nsteps != 4)) {
+    g_err = "dwt_gpu_col_segment_rows: bad argument";
+    return DWT_GPU_EINVAL;
+  }
+  std::vector<int> r, w;
+  col_segment_access(forward != 0, n, nsteps, seg, r, w);
+  if ((int)r.size() > max_reads || (int)w.size() > max_writes) {
+    g_err = "dwt_gpu_col_segment_rows: output arrays too small";
+    return DWT_GPU_EINVAL;
+  }
+  std::copy(r.begin(), r.end(), reads);
+  std::copy(w.begin(), w.end(), writes);
+  *nreads = (int)r.size();
+  *nwrites = (int)w.size();
+  return DWT_GPU_OK;
+}
+
 int dwt_gpu_mid_set(long long max_level_bytes) {
   if (max_level_bytes < 0) {
     g_err = "dwt_gpu_mid_set: negative size";

@@ -1,6 +1,8 @@
 // Host-side launch interface of the device kernels.
 #pragma once
 
+#include <vector>
+
 #include "dwt_common.h"
 
 namespace dwt {
@@ -209,6 +211,8 @@ void launch_col_level(bool fwd, const T* a, long long ap, const T* b, long long 
 // [R s, R (s + 1)) (R = col_segment_rows(), NS = lifting steps).
 int col_segments(int n);
 int col_segment_rows();
+void col_segment_access(bool fwd, int n, int nsteps, int seg, std::vector<int>& reads,
+                        std::vector<int>& writes);
 template <typename T>
 void launch_col_segments(bool fwd, const T* a, long long ap, const T* b, long long bp, T* c,
                          long long cp, int n, int count, int seg0, int nseg, int hbase,

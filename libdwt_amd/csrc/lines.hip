@@ -14,6 +14,9 @@
 // The arithmetic is the reference's line lifting (src/libdwt.c:5512-5603,
 // 5779-5870): updates x + c * (l + r) as separate operations, (2c) * n at a
 // line end, scaling after (forward) or before (inverse) the lifting.
+#include <algorithm>
+#include <vector>
+
 #include "tile_body.h"
 #include "kernels.h"
 
@@ -265,6 +268,15 @@ __device__ __forceinline__ void st_pair(T* p, T a, T b, bool va, bool vb, bool v
   }
 }
 
+// The u-th row (u < 2 (SS / 2 + NS)) a column segment `seg` of a level on n
+// rows loads: forward, an input row; inverse, an interleaved row (odd: H row
+// r >> 1, even: L row r >> 1).  Mirrored at the ends.  Shared with the host
+// (dwt_gpu_col_segment_rows), so the staged in-place level 0 is checked
+// against exactly these rows.
+__host__ __device__ inline int col_load_row(int seg, int u, int n, int SS, int NS) {
+  return mirror_near(2 * (((seg * SS) >> 1) - NS / 2) + u, n);
+}
+
 // A signal is a column: sample i of signal x at row i.  Forward level on n
 // rows: src (row pitch sp) -> L rows to ll (pitch llp), H row m to det row
 // hbase + m (pitch dp; hbase = ceil(n/2) puts H in place).  count signals;
@@ -297,7 +309,7 @@ col_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ ll, long
   T ra[2 * NP], rb[2 * NP];
 #pragma unroll
   for (int u = 0; u < 2 * NP; ++u) {
-    const Gbl<T>* row = s + (long long)mirror_near(2 * pbeg + u, n) * sp + x;
+    const Gbl<T>* row = s + (long long)col_load_row(seg, u, n, SS, NS) * sp + x;
     if (VEC && vb) {
       using V2 = typename Vec2<T>::type;
       const V2 v = *(const Gbl<V2>*)row;
@@ -355,7 +367,7 @@ col_inv_kernel(const T* __restrict__ ll, long long llp, const T* __restrict__ de
   T ra[2 * NP], rb[2 * NP];
 #pragma unroll
   for (int u = 0; u < 2 * NP; ++u) {
-    const int Yr = mirror_near(2 * pbeg + u, n);
+    const int Yr = col_load_row(seg, u, n, SS, NS);
     const int m = Yr >> 1;
     const Gbl<T>* row = (Yr & 1) ? hp + (long long)(hbase + m) * dp + x : lp + (long long)m * llp + x;
     if (VEC && vb) {
@@ -522,6 +534,34 @@ void launch_col_level(bool fwd, const T* a, long long ap, const T* b, long long 
 
 int col_segments(int n) { return (n + kColRows - 1) / kColRows; }
 int col_segment_rows() { return kColRows; }
+
+// Rows one segment reads and writes (the column kernels' index arithmetic).
+// Forward: reads input rows, writes H rows m (at hbase + m).  Inverse: reads
+// H rows m (at hbase + m; its L rows come from scratch), writes output rows.
+// Each list sorted, without duplicates; returns the counts.
+void col_segment_access(bool fwd, int n, int nsteps, int seg, std::vector<int>& reads,
+                        std::vector<int>& writes) {
+  const int SS = kColRows, NS = nsteps, NP = SS / 2 + NS;
+  reads.clear();
+  writes.clear();
+  for (int u = 0; u < 2 * NP; ++u) {
+    const int r = col_load_row(seg, u, n, SS, NS);
+    if (fwd)
+      reads.push_back(r);
+    else if (r & 1)
+      reads.push_back(r >> 1);
+  }
+  if (fwd) {
+    const int half = (n + 1) >> 1, nh = n >> 1;
+    const int m0 = (seg * SS) >> 1, m1 = std::min(m0 + SS / 2, half);
+    for (int m = m0; m < m1; ++m)
+      if (m < nh) writes.push_back(m);
+  } else {
+    for (int r = seg * SS; r < std::min(seg * SS + SS, n); ++r) writes.push_back(r);
+  }
+  std::sort(reads.begin(), reads.end());
+  reads.erase(std::unique(reads.begin(), reads.end()), reads.end());
+}
 
 template void launch_col_segments<float>(bool, const float*, long long, const float*, long long, float*,
                                          long long, int, int, int, int, int, const Coeffs<float>&, bool,

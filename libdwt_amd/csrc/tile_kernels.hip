@@ -141,10 +141,13 @@ static TileShape pick_tile(int H, int nstrips, bool fwd) {
   const TileShape t = g_tiles[fwd ? 0 : 1][height_class(H)];
   if (t.S > 0) return t;
   // default from autotuning sweeps on MI355X (4096^2 and 3001x2047, both
-  // wavelets, both directions): 8-row segments loaded in a single shot were
-  // fastest or within noise on every level size
+  // wavelets, both directions, with the level-0 detail copy riding on the
+  // small levels): 8-row segments loaded in a single shot, except 4-row ones
+  // for the forward levels of at most 1024 rows and the inverse levels of at
+  // most 512 rows, whose shorter tile chains finish a latency-bound level
+  // sooner (0.5-1.1 us per level)
   (void)nstrips;
-  (void)H;
+  if (H <= (fwd ? 1024 : 512)) return TileShape{4, 0};
   return TileShape{8, 0};
 }
 

@@ -516,4 +516,128 @@ void dwt_util_abort() {
   std::abort();
 }
 
+// ceil(log2 x): the bit length of pow2ceil(x) - 1 (src/libdwt.c:618-627);
+// x must be positive (the reference asserts it, src/libdwt.c:583).
+int dwt_util_ceil_log2(int x) {
+  if (x <= 0) {
+    dwt_util_log(LOG_ERR, "dwt_util_ceil_log2: x must be positive\n");
+    dwt_util_abort();
+  }
+  int r = 0;
+  while ((1ll << r) < (long long)x) ++r;
+  return r;
+}
+
+// Every timer type this library implements is available (src/libdwt.c:7259):
+// the clock_gettime family, clock, times, getrusage, gettimeofday, and the
+// RTC stand-in above.
+int dwt_util_clock_available(int type) {
+  switch (type) {
+    case DWT_TIME_CLOCK_GETTIME:
+    case DWT_TIME_CLOCK_GETTIME_REALTIME:
+    case DWT_TIME_CLOCK_GETTIME_MONOTONIC:
+    case DWT_TIME_CLOCK_GETTIME_MONOTONIC_RAW:
+    case DWT_TIME_CLOCK_GETTIME_PROCESS_CPUTIME_ID:
+    case DWT_TIME_CLOCK_GETTIME_THREAD_CPUTIME_ID:
+    case DWT_TIME_CLOCK:
+    case DWT_TIME_TIMES:
+    case DWT_TIME_GETRUSAGE:
+    case DWT_TIME_GETRUSAGE_SELF:
+    case DWT_TIME_GETRUSAGE_CHILDREN:
+    case DWT_TIME_GETRUSAGE_THREAD:
+    case DWT_TIME_GETTIMEOFDAY:
+    case DWT_TIME_IOCTL_RTC:
+      return 0;
+    default:
+      return -1;
+  }
+}
+
+// No ASVP workers here (src/libdwt.c:7407-7414).
+int dwt_util_get_max_workers() { return 1; }
+
+// Busy-wait on the automatically selected clock (src/libdwt.c:7375-7386).
+void dwt_util_wait(int ms) {
+  if (ms <= 0) {
+    dwt_util_log(LOG_ERR, "dwt_util_wait: ms must be positive\n");
+    dwt_util_abort();
+  }
+  const int type = dwt_util_clock_autoselect();
+  const double freq = (double)dwt_util_get_frequency(type);
+  const dwt_clock_t start = dwt_util_get_clock(type);
+  while (1000.0 * (double)(dwt_util_get_clock(type) - start) / freq < (double)ms) {
+  }
+}
+
+// Float vectors (src/libdwt.c:7689-7830).  A pair of values is "different"
+// when either is not normal-or-zero or they differ by more than 1e-4; the
+// difference is logged at LOG_ERR as the reference does.
+static int vec_differ(float a, float b) {
+  const auto ok = [](float v) { return v == 0.0f || std::isnormal(v); };
+  if (!ok(a) || !ok(b)) {
+    dwt_util_log(LOG_ERR, "%f or %f is not normal nor zero!\n", a, b);
+    return 1;
+  }
+  if (std::fabs(a - b) > 1e-4f) {
+    dwt_util_log(LOG_ERR, "%f should be %f!\n", a, b);
+    return 1;
+  }
+  return 0;
+}
+
+float* dwt_util_allocate_vec_s(int size) {
+  if (size < 0 || (size & 1)) {
+    dwt_util_log(LOG_ERR, "dwt_util_allocate_vec_s: size must be even\n");
+    dwt_util_abort();
+  }
+  void* p = nullptr;
+  if (posix_memalign(&p, 8, sizeof(float) * (size_t)(size > 0 ? size : 2)) != 0) return nullptr;
+  return static_cast<float*>(p);
+}
+
+int dwt_util_generate_vec_s(float* addr, int size) {
+  for (int i = 0; i < size; ++i) addr[i] = (float)i;
+  for (int i = 0; i < size; ++i)
+    if (vec_differ(addr[i], (float)i)) return 1;
+  return 0;
+}
+
+int dwt_util_zero_vec_s(float* addr, int size) {
+  for (int i = 0; i < size; ++i) addr[i] = 0.0f;
+  for (int i = 0; i < size; ++i)
+    if (vec_differ(addr[i], 0.0f)) return 1;
+  return 0;
+}
+
+int dwt_util_copy_vec_s(const float* src, float* dst, int size) {
+  if (size > 0) std::memmove(dst, src, sizeof(float) * (size_t)size);
+  for (int i = 0; i < size; ++i)
+    if (vec_differ(dst[i], src[i])) return 1;
+  return 0;
+}
+
+int dwt_util_cmp_vec_s(const float* a, const float* b, int size) {
+  for (int i = size - 1; i >= 0; --i)
+    if (vec_differ(a[i], b[i])) return 1;
+  return 0;
+}
+
+// printf replacements (src/libdwt.c:7921-7951).
+int dwt_util_vfprintf(FILE* stream, const char* format, va_list ap) { return std::vfprintf(stream, format, ap); }
+int dwt_util_vprintf(const char* format, va_list ap) { return dwt_util_vfprintf(stdout, format, ap); }
+int dwt_util_fprintf(FILE* stream, const char* format, ...) {
+  va_list ap;
+  va_start(ap, format);
+  const int r = dwt_util_vfprintf(stream, format, ap);
+  va_end(ap);
+  return r;
+}
+int dwt_util_printf(const char* format, ...) {
+  va_list ap;
+  va_start(ap, format);
+  const int r = dwt_util_vprintf(format, ap);
+  va_end(ap);
+  return r;
+}
+
 }  // extern "C"

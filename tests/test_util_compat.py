@@ -112,6 +112,62 @@ def test_conv_show_pgm_subband_addr(libs):
         assert L.dwt_util_addr_coeff_s(buf.ctypes.data, 3, 5, 400, 4) == buf.ctypes.data + 3 * 400 + 20
 
 
+@need_ref
+def test_more_utilities_match_reference(libs):
+    """ceil_log2, worker count and the float-vector helpers, call for call;
+    clock availability agrees except that this library also implements
+    CLOCK_MONOTONIC_RAW, which the reference build leaves out."""
+    ours, ref = libs
+    for x in list(range(1, 5000)) + [1 << 20, (1 << 20) + 1, 1 << 30]:
+        assert ours.dwt_util_ceil_log2(x) == ref.dwt_util_ceil_log2(x), x
+    assert ours.dwt_util_get_max_workers() == ref.dwt_util_get_max_workers() == 1
+    for t in range(16):
+        got, want = ours.dwt_util_clock_available(t), ref.dwt_util_clock_available(t)
+        assert got == want or (t == 3 and got == 0), t
+    for L in (ours, ref):
+        L.dwt_util_allocate_vec_s.restype = C.c_void_p
+        for f in ("generate", "zero"):
+            getattr(L, f"dwt_util_{f}_vec_s").argtypes = [C.c_void_p, C.c_int]
+        for f in ("copy", "cmp"):
+            getattr(L, f"dwt_util_{f}_vec_s").argtypes = [C.c_void_p, C.c_void_p, C.c_int]
+    res = []
+    for L in (ours, ref):
+        a = np.zeros(40, np.float32)
+        b = np.zeros(40, np.float32)
+        out = [L.dwt_util_generate_vec_s(a.ctypes.data, 40), L.dwt_util_zero_vec_s(b.ctypes.data, 40),
+               L.dwt_util_cmp_vec_s(a.ctypes.data, b.ctypes.data, 40), L.dwt_util_copy_vec_s(a.ctypes.data, b.ctypes.data, 40),
+               L.dwt_util_cmp_vec_s(a.ctypes.data, b.ctypes.data, 40)]
+        b[9] = np.float32(1e-40)  # denormal: "not normal nor zero"
+        out.append(L.dwt_util_cmp_vec_s(a.ctypes.data, b.ctypes.data, 40))
+        b[9] = a[9] + np.float32(5e-5)
+        out.append(L.dwt_util_cmp_vec_s(a.ctypes.data, b.ctypes.data, 40))
+        out += [a.tobytes(), b.tobytes()]
+        res.append(out)
+    assert res[0] == res[1]
+
+
+@need_ref
+def test_c_program_using_util_helpers_links_and_matches_reference():
+    """A C program calling dwt_util_ceil_log2, dwt_util_printf and the other
+    helpers links unchanged against libdwt_gpu.so and prints exactly what it
+    prints linked against the reference."""
+    src = os.path.join(os.path.dirname(__file__), "c", "util_helpers.c")
+    inc = os.path.dirname(libdwt_amd.HEADER)
+    outs = []
+    with tempfile.TemporaryDirectory() as d:
+        for name, so in (("ours", libdwt_amd.LIB_PATH), ("ref", orc.REF_SO)):
+            exe = os.path.join(d, name)
+            r = subprocess.run(["gcc", "-std=gnu99", "-Wall", "-Werror=implicit-function-declaration", "-I", inc,
+                                src, so, f"-Wl,-rpath,{os.path.dirname(so)}", "-lm", "-o", exe],
+                               capture_output=True, text=True)
+            assert r.returncode == 0, r.stderr[-2000:]
+            r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+            assert r.returncode == 0, r.stderr[-2000:]
+            outs.append(r.stdout)
+    assert outs[0] == outs[1]
+    assert "ceil_log2(70)=7" in outs[0] and "vprintf 42" in outs[0]
+
+
 @pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="reference examples not present")
 @pytest.mark.parametrize("example", ["start", "simple", "simple-double", "simple-perf", "subbands",
                                      "perf-plot"])
