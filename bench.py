@@ -17,6 +17,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -26,20 +27,30 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fwd_level_4096x4096_cdf97.md")
+
+
+STEP_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_step_4096x4096_cdf97_J8.md")
+
+
+def profile_value(path: str, key: str):
+    """An integer `key: value` line of a committed profile summary (None when
+    the file or the line is missing)."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith(key + ":"):
+                return int(line.split(":")[1].split()[0])
+    return None
 
 
 def measured_traffic(width: int, height: int, wavelet: str):
     """HBM bytes per launch of the level-0 forward kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain copy);
     None when no summary exists for this workload."""
-    if (width, height, wavelet) != (4096, 4096, "cdf97") or not os.path.exists(TRAFFIC_PROFILE):
+    if (width, height, wavelet) != (4096, 4096, "cdf97"):
         return None
-    with open(TRAFFIC_PROFILE) as fh:
-        for line in fh:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1].split()[0])
-    return None
+    return profile_value(STEP_TRAFFIC_PROFILE, "level0_fwd_traffic_bytes")
 
 
 def bench_sparse(args, torch, libdwt_amd, workloads) -> None:
@@ -132,6 +143,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="image workload: skip the batched / serving / drop-in 1D legs")
     ap.add_argument("--schedule", type=int, default=0, choices=(0, 1),
                     help="levels after level 0: 1 one pyramid launch, 0 one launch per level")
     ap.add_argument("--cold-steps", type=int, default=10,
@@ -144,9 +157,9 @@ def parse():
                          "images of uneven sizes, batched launches vs one launch per image; "
                          "serve: Poisson request stream through the serving loop; lines: "
                          "16384 random walks of 8192 samples, full-depth 1D")
-    ap.add_argument("--rate", type=float, default=8000.0,
+    ap.add_argument("--rate", type=float, default=30000.0,
                     help="serve: offered request rate (requests/s)")
-    ap.add_argument("--requests", type=int, default=4000, help="serve: requests per run")
+    ap.add_argument("--requests", type=int, default=3000, help="serve: requests per run")
     ap.add_argument("--client", default="native", choices=("native", "python"),
                     help="serve: submit from the library's native replay loop or from Python")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -155,46 +168,50 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(img: np.ndarray, wavelet: str, levels: int, budget_s: float) -> dict:
-    """Reference CPU path (fwd + inv of the same image), min over repeats, on
-    this host's cores; bounded by budget_s of wall time."""
-    from oracle import oracle as orc
-    threads = min(os.cpu_count() or 1, 16)
-    if orc.have_ref():
-        lib = orc.Reference()
-        lib.set_threads(threads)
-        kind = "reference"
-        modes = (9, 0)
-    else:
-        lib = orc.oracle()
-        lib.set_threads(threads)
-        kind = "port"
-        modes = (None,)
-    H, W = img.shape
-    buf = np.empty_like(img)
-    best = float("inf")
-    best_mode = None
-    reps = 0
-    per_mode = budget_s / len(modes)
-    for mode in modes:
-        if mode is not None:
-            lib.set_accel(mode)
-        t_end = time.perf_counter() + per_mode
-        n = 0
-        while n < 2 or (time.perf_counter() < t_end and n < 100):
-            buf[...] = img
-            t0 = time.perf_counter()
-            j = lib.fwd2_raw(buf, wavelet, 4 * W, 4, W, H, W, H, levels)
-            lib.inv2_raw(buf, wavelet, 4 * W, 4, W, H, W, H, j)
-            dt = time.perf_counter() - t0
-            n += 1
-            if dt < best:
-                best, best_mode = dt, mode
-        reps += n
-    sample = (f"{W}x{H} {wavelet} J={levels} fwd+inv, min of {reps} runs"
-              + (f", accel {best_mode} (fastest of {list(modes)})" if kind == "reference" else ""))
-    return {"value": round(W * H / best / 1e6, 3), "unit": "Mpix/s", "cores": threads,
-            "kind": kind, "sample": sample, "seconds_per_step": best}
+def host_cores() -> dict:
+    """Logical CPUs this process may use and the physical cores behind them."""
+    logical = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    phys = None
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=10).stdout
+        phys = len({ln for ln in out.splitlines() if ln and not ln.startswith("#")})
+    except Exception:  # lscpu missing: report the logical count only
+        phys = None
+    return {"logical": logical, "physical": phys}
+
+
+def cpu_run(width, height, wavelet, levels, threads, seconds) -> dict:
+    """One run of oracle/cpu_bench.py (the reference's drivers on this host) in
+    a child process, OMP_PROC_BIND=close and OMP_NUM_THREADS=threads."""
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_NUM_THREADS=str(threads))
+    cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_bench.py"), "--width", str(width), "--height",
+           str(height), "--wavelet", wavelet, "--levels", str(levels), "--threads", str(threads),
+           "--seconds", str(seconds)]
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=seconds * 4 + 120)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu_bench failed: {r.stderr[-500:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(width, height, wavelet, levels, budget_s: float) -> dict:
+    """The reference CPU path (oracle/_ref: the reference built from its own
+    sources; the clean-room port when it is absent) on this host: all physical
+    cores (at most 16) and one core, OMP_PROC_BIND=close, forward and inverse
+    timed separately, minimum over repeated runs, the faster accel mode."""
+    cores = host_cores()
+    nthreads = max(1, min(16, cores["physical"] or cores["logical"], cores["logical"]))
+    multi = cpu_run(width, height, wavelet, levels, nthreads, 0.6 * budget_s)
+    single = cpu_run(width, height, wavelet, levels, 1, 0.4 * budget_s)
+    W, H = width, height
+    sample = (f"{W}x{H} {wavelet} J={levels} fwd then inv on one image, min of {multi['runs']} runs "
+              f"({nthreads} threads) / {single['runs']} runs (1 thread), OMP_PROC_BIND=close"
+              + (f", accel {multi['accel']} (faster of [9, 0])" if multi["kind"] == "reference" else ""))
+    return {"value": round(multi["Mpix_s"], 3), "unit": "Mpix/s", "cores": nthreads, "kind": multi["kind"],
+            "sample": sample,
+            "fwd_ms": round(1e3 * multi["fwd_s"], 3), "inv_ms": round(1e3 * multi["inv_s"], 3),
+            "single_thread": {"value": round(single["Mpix_s"], 3), "fwd_ms": round(1e3 * single["fwd_s"], 3),
+                              "inv_ms": round(1e3 * single["inv_s"], 3), "accel": single["accel"]},
+            "host_cores": cores}
 
 
 def _timed(torch, stream, fn, steps):
@@ -210,16 +227,16 @@ def _timed(torch, stream, fn, steps):
     return e0.elapsed_time(e1) / steps
 
 
-def bench_batch(args, torch, libdwt_amd, workloads):
+def measure_batch(wv, steps, warm, torch, libdwt_amd, workloads, storages=("f32", "f16", "bf16"), graph=True,
+                  nerr=32):
     """256 float32 images, each side log-uniform in [64, 2048] (seed 4),
     J = min(8, floor(log2(min side))), forward then inverse: batched launches
     (FP32 / FP16 / BF16 coefficient storage, eagerly and as a replayed HIP
     graph) against one drop-in transform per image.  Errors are measured
     against the library's own FP64 transform (bit-identical to the reference's
-    _d path) on the first 32 images."""
+    _d path) on the first nerr images."""
     import ctypes
     import math
-    wv = args.wavelet
     sizes = workloads.uneven_sizes(256, 64, 2048, seed=4)
     levels = [min(8, int(math.floor(math.log2(min(w, h))))) for w, h in sizes]
     area = sum(w * h for w, h in sizes)
@@ -229,7 +246,7 @@ def bench_batch(args, torch, libdwt_amd, workloads):
     stream = torch.cuda.Stream(device=dev)
     sp = stream.cuda_stream
     L = libdwt_amd.lib()
-    steps, warm = max(1, args.steps), max(1, args.warmup)
+    steps, warm = max(1, steps), max(1, warm)
 
     # one drop-in style transform per image (in place on working copies)
     work = [t.clone() for t in imgs]
@@ -246,7 +263,6 @@ def bench_batch(args, torch, libdwt_amd, workloads):
     del work
 
     # FP64 reference coefficients of the first images (the GPU's own _d path)
-    nerr = 32
     ref64 = []
     with torch.cuda.stream(stream):  # conversion and transform ordered on one stream
         for t, j in zip(imgs[:nerr], levels[:nerr]):
@@ -269,7 +285,7 @@ def bench_batch(args, torch, libdwt_amd, workloads):
         return m
 
     modes = {}
-    for storage in ("f32", "f16", "bf16"):
+    for storage in storages:
         dt = {"f32": torch.float32, "f16": torch.float16, "bf16": torch.bfloat16}[storage]
         coefs = [torch.empty(t.shape, dtype=dt, device=dev) for t in imgs]
         outs = [torch.empty_like(t) for t in imgs]
@@ -291,13 +307,15 @@ def bench_batch(args, torch, libdwt_amd, workloads):
         for _ in range(warm):
             step()
         ms = _timed(torch, stream, step, steps)
-        # the same two plan executions captured in a HIP graph and replayed
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g, stream=stream):
-            step()
-        with torch.cuda.stream(stream):  # replay() launches on the current stream
-            g.replay()
-            ms_graph = _timed(torch, stream, g.replay, steps)
+        ms_graph, g = None, None
+        if graph:
+            # the same two plan executions captured in a HIP graph and replayed
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=stream):
+                step()
+            with torch.cuda.stream(stream):  # replay() launches on the current stream
+                g.replay()
+                ms_graph = _timed(torch, stream, g.replay, steps)
         # per-level kernel times of one eager step
         libdwt_amd.profile_reset()
         libdwt_amd.profile_enable(True)
@@ -330,14 +348,26 @@ def bench_batch(args, torch, libdwt_amd, workloads):
         del g
         modes[storage] = {
             "ms_per_step": round(ms, 4), "Mpix_s": round(area / (ms * 1e-3) / 1e6, 1),
-            "ms_per_step_graph": round(ms_graph, 4),
-            "Mpix_s_graph": round(area / (ms_graph * 1e-3) / 1e6, 1),
             "coef_max_abs_err_vs_fp64": cerr, "recon_max_abs_err": rerr,
             "recon_psnr_min_db": round(psnr, 2), "level_kernels_us": kern,
         }
+        if ms_graph is not None:
+            modes[storage]["ms_per_step_graph"] = round(ms_graph, 4)
+            modes[storage]["Mpix_s_graph"] = round(area / (ms_graph * 1e-3) / 1e6, 1)
     alg = sum(2 * workloads.algorithmic_bytes(w, h, j) for (w, h), j in zip(sizes, levels))
     f32 = modes["f32"]
-    best_ms = min(f32["ms_per_step"], f32["ms_per_step_graph"])
+    best_ms = min(f32["ms_per_step"], f32.get("ms_per_step_graph", f32["ms_per_step"]))
+    return {"sizes": sizes, "area": area, "alg_bytes": alg, "plain_ms": plain_ms, "best_ms": best_ms,
+            "modes": modes, "steps": steps, "warm": warm}
+
+
+def bench_batch(args, torch, libdwt_amd, workloads):
+    """The batch workload on its own (see measure_batch)."""
+    wv = args.wavelet
+    r = measure_batch(wv, args.steps, args.warmup, torch, libdwt_amd, workloads)
+    sizes, area, alg, plain_ms, best_ms, modes = (r["sizes"], r["area"], r["alg_bytes"], r["plain_ms"],
+                                                   r["best_ms"], r["modes"])
+    steps, warm = r["steps"], r["warm"]
     out = {
         "metric": f"Mpix/s, 256 uneven images (sides 64..2048) {wv} batched fwd+inv",
         "value": round(area / (best_ms * 1e-3) / 1e6, 1),
@@ -357,8 +387,8 @@ def bench_batch(args, torch, libdwt_amd, workloads):
     print(json.dumps(out), flush=True)
 
 
-def bench_serve(args, torch, libdwt_amd, workloads):
-    """Serving: a Poisson stream of requests at --rate requests/s; each request
+def measure_serve(rate, n, client, torch, libdwt_amd, workloads):
+    """Serving: a Poisson stream of requests at `rate` requests/s; each request
     is one float32 image with sides log-uniform in [128, 4096] (64 distinct
     resident images, seed 7), CDF 9/7 or 5/3 (50/50), forward or inverse
     (50/50; an inverse request reads the image's samples as coefficients),
@@ -373,7 +403,6 @@ def bench_serve(args, torch, libdwt_amd, workloads):
     dev = torch.device("cuda:0")
     pool = [torch.from_numpy(workloads.sinusoid_image(w, h, seed=2000 + i)).to(dev)
             for i, (w, h) in enumerate(sizes)]
-    n = args.requests
     reqs = []
     for _ in range(n):
         k = int(rng.integers(len(pool)))
@@ -381,7 +410,7 @@ def bench_serve(args, torch, libdwt_amd, workloads):
         jmax = min(8, workloads.ceil_log2(min(w, h)))
         reqs.append((k, "cdf97" if rng.random() < 0.5 else "cdf53", bool(rng.random() < 0.5),
                      int(rng.integers(1, jmax + 1))))
-    gaps = rng.exponential(1.0 / args.rate, n)
+    gaps = rng.exponential(1.0 / rate, n)
     outs = [torch.empty_like(pool[k]) for k, _, _, _ in reqs]
     area = sum(pool[k].numel() for k, _, _, _ in reqs)
     torch.cuda.synchronize()
@@ -394,7 +423,7 @@ def bench_serve(args, torch, libdwt_amd, workloads):
             s.drain()
             s.poll(1 << 16)
             b0, _ = s.stats()
-            if args.client == "native":
+            if client == "native":
                 rl = [s.request(wv, fwd, pool[k], out, j) for (k, wv, fwd, j), out in zip(reqs, outs)]
                 at = np.cumsum(gaps).tolist()
                 t0 = time.perf_counter()
@@ -421,6 +450,13 @@ def bench_serve(args, torch, libdwt_amd, workloads):
 
     batched = run(256, 4)
     plain = run(1, 1)
+    return batched, plain
+
+
+def bench_serve(args, torch, libdwt_amd, workloads):
+    """The serving workload on its own (see measure_serve)."""
+    n = args.requests
+    batched, plain = measure_serve(args.rate, n, args.client, torch, libdwt_amd, workloads)
     out = {
         "metric": "serving latency p50/p99 and Mpix/s, Poisson stream, uneven images 128..4096",
         "value": batched["Mpix_s"], "unit": "Mpix/s",
@@ -510,6 +546,54 @@ def bench_lines(args, torch, libdwt_amd, workloads):
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_lines_baseline(sig, wv, args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def measure_lines_dropin(torch, libdwt_amd, workloads, wavelet: str, count: int = 64, N: int = 8192) -> dict:
+    """The reference's own caller loop for a full-depth 1D transform
+    (src/libdwt.h:219-348: one dwt_cdf97_f_ex_stride_s call per signal and
+    level, L to [0, ceil(n/2)) and H after it in place; then the inverse
+    calls coarsest level first) on `count` random-walk signals of N samples in
+    device memory, against dwt_gpu_1d_batch on the same signals.  Each drop-in
+    call is synchronous, as the reference's are."""
+    import ctypes
+    L = libdwt_amd.lib()
+    sig = workloads.random_walk(count, N, seed=11)
+    t = torch.from_numpy(sig).cuda()
+    base = t.data_ptr()
+    f = getattr(L, f"dwt_{wavelet}_f_ex_stride_s")
+    i = getattr(L, f"dwt_{wavelet}_i_ex_stride_s")
+    sizes = []
+    n = N
+    while n >= 2:
+        sizes.append(n)
+        n = (n + 1) // 2
+    torch.cuda.synchronize()
+    calls = 0
+    t0 = time.perf_counter()
+    for k in range(count):
+        p = base + 4 * N * k
+        for n in sizes:
+            f(ctypes.c_void_p(p), ctypes.c_void_p(p), ctypes.c_void_p(p + 4 * ((n + 1) // 2)), None, n, 4)
+            calls += 1
+        for n in reversed(sizes):
+            i(ctypes.c_void_p(p), ctypes.c_void_p(p + 4 * ((n + 1) // 2)), ctypes.c_void_p(p), None, n, 4)
+            calls += 1
+    dt = time.perf_counter() - t0
+    err = float(np.abs(t.cpu().numpy() - sig).max())
+    stream = torch.cuda.Stream()
+    sp = stream.cuda_stream
+
+    def batch_step():
+        j = libdwt_amd.lines_1d(wavelet, True, base, 4 * N, 4, N, count, -1, stream=sp)
+        libdwt_amd.lines_1d(wavelet, False, base, 4 * N, 4, N, count, j, stream=sp)
+
+    batch_step()
+    ms = _timed(torch, stream, batch_step, 20)
+    return {"signals": count, "length": N, "levels": len(sizes), "calls": calls,
+            "us_per_call": round(1e6 * dt / calls, 2),
+            "Msamples_s": round(count * N / dt / 1e6, 2),
+            "batch_1d_Msamples_s": round(count * N / (ms * 1e-3) / 1e6, 1),
+            "max_abs_roundtrip_err": err}
 
 
 def cpu_lines_baseline(sig: np.ndarray, wavelet: str, budget_s: float) -> dict:
@@ -613,21 +697,37 @@ def main():
     # cold steps: the image and scratch (~150 MB) fit the 256 MB Infinity
     # Cache, so back-to-back steps are partly served from it; here each step
     # runs alone after writing a 512 MiB buffer (untimed) to evict them
-    cold = []
+    cold, cold_l0_us = [], None
     if args.cold_steps > 0:
         scrub = torch.empty(512 << 18, device=dev)
-        for _ in range(args.cold_steps):
-            with torch.cuda.stream(stream):
-                scrub.fill_(1.0)
-            c0 = torch.cuda.Event(enable_timing=True)
-            c1 = torch.cuda.Event(enable_timing=True)
-            c0.record(stream)
-            step()
-            c1.record(stream)
-            torch.cuda.synchronize()
-            cold.append(c0.elapsed_time(c1))
-        del scrub
+
+        def cold_steps():
+            for _ in range(args.cold_steps):
+                with torch.cuda.stream(stream):
+                    scrub.fill_(1.0)
+                c0 = torch.cuda.Event(enable_timing=True)
+                c1 = torch.cuda.Event(enable_timing=True)
+                c0.record(stream)
+                step()
+                c1.record(stream)
+                torch.cuda.synchronize()
+                cold.append(c0.elapsed_time(c1))
+
+        cold_steps()
         cold.sort()
+        step_times = cold
+        # again with HIP events around every launch: the level-0 kernel alone
+        # after the scrub
+        cold = []
+        libdwt_amd.profile_reset()
+        libdwt_amd.profile_enable(True)
+        cold_steps()
+        libdwt_amd.profile_enable(False)
+        n0, ms0 = libdwt_amd.profile_read(0, 0)
+        cold_l0_us = 1000 * ms0 / n0 if n0 else None
+        libdwt_amd.profile_reset()
+        cold = step_times
+        del scrub
 
     # per-kernel timing pass (same steps, HIP events around every launch)
     libdwt_amd.profile_reset()
@@ -684,8 +784,13 @@ def main():
 
     mpix = W * H / (ms_per_step * 1e-3) / 1e6
     alg_step = workloads.algorithmic_bytes(W, H, J) * 2
+    headline = (W, H, J, wv) == (4096, 4096, 8, "cdf97")
+    metric = ("Mpix/s + %HBM peak, 4096\u00b2 CDF9/7 J=8 fwd+inv; batched p50/p99 latency" if headline else
+              f"Mpix/s, {W}x{H} {'CDF9/7' if wv == 'cdf97' else 'CDF5/3'} J={J} fwd+inv")
+    step_traffic = profile_value(STEP_TRAFFIC_PROFILE, "step_traffic_bytes") if headline else None
+    lv0_single = lv0["avg_us"] if lv0 else None
     out = {
-        "metric": f"Mpix/s, {W}x{H} {'CDF9/7' if wv == 'cdf97' else 'CDF5/3'} J={J} fwd+inv",
+        "metric": metric,
         "value": round(mpix, 3),
         "unit": "Mpix/s",
         "n_gpus": 1,
@@ -710,8 +815,18 @@ def main():
             "traffic": measured_traffic(W, H, wv),
             "algorithmic_bytes": alg_bytes,
             "avg_launch_us": round(lv0_us, 3) if lv0_us else None,
-            "avg_launch_us_single_events": lv0["avg_us"] if lv0 else None,
+            "avg_launch_us_single_events": lv0_single,
             "launches_timed": n_rep,
+            # the same kernel right after a 512 MiB scrub (nothing of the
+            # image in the 256 MB Infinity Cache), one launch between events,
+            # beside the warm single-launch figure timed the same way
+            "cold_single_launch_us": round(cold_l0_us, 3) if cold_l0_us else None,
+            "achieved_cold": round(alg_bytes / (cold_l0_us * 1e-6) / 1e9, 1) if cold_l0_us else None,
+            "frac_cold": round(alg_bytes / (cold_l0_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if cold_l0_us else None,
+            "hbm_frac_step": round(alg_step / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "step_algorithmic_bytes": alg_step,
+            "step_traffic": step_traffic,
+            "step_traffic_over_algorithmic": round(step_traffic / alg_step, 3) if step_traffic else None,
             "copy_ceiling": {"gbs": round(copy_gbs, 1), "avg_us": round(copy_us, 3),
                              "bytes": 8 * W * H,
                              "frac_of_copy": round(achieved / copy_gbs, 4) if achieved else None},
@@ -724,10 +839,27 @@ def main():
         "wall_s": round(wall, 4),
         "max_abs_roundtrip_err": rec_err,
     }
+    if headline and not args.no_batched:
+        # the metric's batched form: 256 uneven images in batched launches
+        # against one transform per image, and the continuous-batching server
+        # on a Poisson stream against one request at a time
+        b = measure_batch(wv, 5, 2, torch, libdwt_amd, workloads, storages=("f32",), graph=False, nerr=8)
+        sb, sp1 = measure_serve(args.rate, args.requests, "native", torch, libdwt_amd, workloads)
+        out["batched"] = {
+            "p50_ms": sb["p50_ms"], "p99_ms": sb["p99_ms"], "Mpix_s": sb["Mpix_s"],
+            "vs_plain": round(sb["Mpix_s"] / sp1["Mpix_s"], 2),
+            "p99_plain_over_batched": round(sp1["p99_ms"] / sb["p99_ms"], 2),
+            "serve": {"rate_per_s": args.rate, "requests": args.requests, "batched": sb,
+                      "one_request_at_a_time": sp1},
+            "batch_256_uneven": {"Mpix_s": round(b["area"] / (b["best_ms"] * 1e-3) / 1e6, 1),
+                                 "ms_per_step": round(b["best_ms"], 4),
+                                 "plain_per_image_Mpix_s": round(b["area"] / (b["plain_ms"] * 1e-3) / 1e6, 1),
+                                 "vs_plain": round(b["plain_ms"] / b["best_ms"], 2),
+                                 "coef_max_abs_err_vs_fp64": b["modes"]["f32"]["coef_max_abs_err_vs_fp64"]},
+        }
+        out["lines_dropin"] = measure_lines_dropin(torch, libdwt_amd, workloads, wv)
     if not args.no_cpu:
-        cb = cpu_baseline(img, wv, J, args.cpu_seconds)
-        cb.pop("seconds_per_step")
-        out["cpu_baseline"] = cb
+        out["cpu_baseline"] = cpu_baseline(W, H, wv, J, args.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 
