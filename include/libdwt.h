@@ -383,6 +383,32 @@ int dwt_gpu_tile_get(int forward, int width, int height, int wavelet, int *seg_r
  * twice that is transformed completely by that one launch. */
 int dwt_gpu_tail_set(long long max_region_bytes);
 
+/* Level 0 of the packed path sends its detail subbands through scratch (one
+ * copy of them per direction, carried by the smaller levels' launches).
+ * inplace = 1 runs it in place instead: each workgroup loads its tile,
+ * signals that its reads are done, waits for the tiles that read what it
+ * overwrites, then stores, and only the rows that close a cycle of that
+ * dependency order pass through scratch (about a sixth of the copy).  Same
+ * results; measured slower on MI355X (the waits cost more than the copy), so
+ * off by default.  band (> 0) sets the row segments per XCD band of the
+ * in-place order (default 8; 0 keeps it).  Launches under stream capture and
+ * the allocation-free entry point always use the scratch route. */
+int dwt_gpu_level0_set(int inplace, int band);
+
+/* Test support (no counterpart in the reference): the in-place level-0 plan of
+ * a width x height image with seg_rows-row tiles (4 or 8).  info[8] receives
+ * {segments, column groups, strips, tiles, dependencies, copied rows,
+ * workgroups, band}; each non-null array receives: order[segments] (the
+ * segment order), segflags[segments] (forward: bit 0 HL rows, bit 1 LH and HH
+ * rows through scratch; inverse: bit 0 band preserved in scratch, bit 8 + u
+ * single-shot read row u of the segment's tiles comes from scratch),
+ * list_off[9] and list[tiles] (tiles of XCD x, tile = segment * groups +
+ * group, for workgroups 8k + x), dep_off[tiles + 1] and deps[dependencies]
+ * (tiles whose reads a tile waits for), rows[copied rows] (image rows the
+ * copy carries). */
+int dwt_gpu_level0_plan(int forward, int width, int height, int wavelet, int seg_rows, int *info, int *order,
+                        int *segflags, int *list_off, int *list, int *dep_off, int *deps, int *rows);
+
 /* Optionally, three middle levels of the packed path run as one launch (each
  * workgroup computes overlapping windows of the three levels in LDS) from the
  * first level j >= 1 whose region is at most max_level_bytes, when all three

@@ -114,6 +114,11 @@ def lib() -> C.CDLL:
         L.dwt_gpu_tile_get.restype = i
         L.dwt_gpu_tail_set.argtypes = [C.c_longlong]
         L.dwt_gpu_tail_set.restype = i
+        L.dwt_gpu_level0_set.argtypes = [i, i]
+        L.dwt_gpu_level0_set.restype = i
+        ip = C.POINTER(C.c_int)
+        L.dwt_gpu_level0_plan.argtypes = [i, i, i, i, i, ip, ip, ip, ip, ip, ip, ip, ip]
+        L.dwt_gpu_level0_plan.restype = i
         L.dwt_gpu_mid_set.argtypes = [C.c_longlong]
         L.dwt_gpu_mid_set.restype = i
         L.dwt_gpu_scratch_bytes.argtypes = [vp]

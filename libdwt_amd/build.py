@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libdwt_gpu.so")
 OBJDIR = os.path.join(HERE, "_build")
-SOURCES = ["line_kernels.hip", "tile_kernels.hip", "block_kernels.hip", "mid_kernels.hip", "pyr_kernels.hip", "batch.hip", "capi.hip",
+SOURCES = ["line_kernels.hip", "tile_kernels.hip", "block_kernels.hip", "inplace.hip", "mid_kernels.hip", "pyr_kernels.hip", "batch.hip", "capi.hip",
            "batch_host.hip", "server.hip", "lines.hip", "util.hip", "verify.hip"]
 ARCH = os.environ.get("DWT_OFFLOAD_ARCH", "gfx950")
 

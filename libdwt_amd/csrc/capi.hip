@@ -139,6 +139,17 @@ struct PyrPlan {
   void* pinned = nullptr;
 };
 
+// A cached in-place level-0 plan of one scratch set: the host plan, its
+// tables in device memory (uploaded from a pinned copy kept for the plan's
+// life, so the upload is stream-ordered) and their offsets.
+struct L0Cached {
+  L0Plan plan;
+  void* dev = nullptr;
+  void* pinned = nullptr;
+  size_t bytes = 0;
+  size_t o_list = 0, o_dep_off = 0, o_deps = 0, o_segflags = 0, o_rows = 0;
+};
+
 struct Scratch {
   // 0: LL of even levels, 1: LL of odd levels, 2: level-0 details,
   // 3: line temp, 4: host staging, 5: LL of the pyramid levels
@@ -155,6 +166,12 @@ struct Scratch {
   bool pyr_used = false;
   unsigned long long* trace = nullptr;  // per-item timeline of the last pyramid launch
   long long trace_n = 0, trace_items = 0;
+  // in-place level 0: plans (most recent first); l0_words = error word,
+  // padding to 256 bytes, then one flag per tile; the last launch's epoch
+  std::vector<L0Cached*> l0_plans;
+  unsigned* l0_words = nullptr;
+  long long l0_cap = 0;
+  unsigned l0_epoch = 0;
   hipStream_t stream = nullptr;
   // side stream for the level-0 detail copies, which touch memory disjoint
   // from what the other levels read and write, so they overlap those levels
@@ -191,7 +208,8 @@ struct Scratch {
     for (const auto& b : buf) s += (long long)b.n;
     for (const PyrPlan* pl : plans)
       s += (long long)(pl->deps_off + sizeof(int2) * (size_t)pl->ncnt);
-    return s + 4 * (4 + sync_cnt);
+    for (const L0Cached* c : l0_plans) s += (long long)c->bytes;
+    return s + 4 * (4 + sync_cnt) + 4 * (l0_words ? 64 + l0_cap : 0);
   }
   void release() {
     for (auto& b : buf) {
@@ -200,6 +218,12 @@ struct Scratch {
     }
     for (PyrPlan* pl : plans) free_plan(pl);
     plans.clear();
+    for (L0Cached* c : l0_plans) free_l0(c);
+    l0_plans.clear();
+    if (l0_words) hipFree(l0_words);
+    l0_words = nullptr;
+    l0_cap = 0;
+    l0_epoch = 0;
     if (sync) hipFree(sync);
     sync = nullptr;
     sync_cnt = 0;
@@ -220,6 +244,11 @@ struct Scratch {
     if (pl->dev) hipFree(pl->dev);
     if (pl->pinned) hipHostFree(pl->pinned);
     delete pl;
+  }
+  static void free_l0(L0Cached* c) {
+    if (c->dev) hipFree(c->dev);
+    if (c->pinned) hipHostFree(c->pinned);
+    delete c;
   }
 };
 
@@ -445,6 +474,19 @@ static void run_pyr(bool fwd, std::vector<PyrTask>& tasks, int S, bool vec, cons
   sc.pyr_used = true;
   LAUNCH(launch_pyr<T>(fwd, vec, S, static_cast<const PyrTask*>(plan->dev), ntasks, nitems, sy, cf,
                        g_pyr_grid_cap, st));
+}
+
+// Reads and clears the in-place level-0 error word of sc (synchronises st);
+// true when a dependency wait of an earlier launch timed out.
+static bool l0_failed(Scratch& sc, hipStream_t st) {
+  if (!sc.l0_words) return false;
+  if (!sc.err_host) HIPCHK(hipHostMalloc((void**)&sc.err_host, sizeof(unsigned), hipHostMallocDefault));
+  HIPCHK(hipMemcpyAsync(sc.err_host, sc.l0_words, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (*sc.err_host == 0) return false;
+  LAUNCH(launch_fill_u32(sc.l0_words, 0u, 1, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return true;
 }
 
 // Reads and clears the pyramid error flag of sc (synchronises st); true when a
@@ -691,6 +733,118 @@ static int mid_start(int W, int H, int jt, int elem) {
   return -1;
 }
 
+// ---- in-place level 0 -----------------------------------------------------------
+// The plan of a level-0 launch (cached per scratch set, most recent first) and
+// its device view with a fresh epoch.
+static int g_l0_inplace = 0;  // off: measured slower than the scratch route on MI355X
+constexpr size_t kL0MaxPlans = 8;
+
+static L0Cached& l0_cached(Scratch& sc, bool fwd, int W, int H, int nsteps, int S, hipStream_t st) {
+  for (size_t i = 0; i < sc.l0_plans.size(); ++i) {
+    L0Cached* c = sc.l0_plans[i];
+    const L0Plan& q = c->plan;
+    if (q.fwd == fwd && q.W == W && q.H == H && q.nsteps == nsteps && q.S == S && q.band == g_l0_band) {
+      sc.l0_plans.erase(sc.l0_plans.begin() + (long)i);
+      sc.l0_plans.insert(sc.l0_plans.begin(), c);
+      return *c;
+    }
+  }
+  L0Cached* c = new L0Cached();
+  c->plan = build_l0_plan(fwd, W, H, nsteps, S);
+  const L0Plan& q = c->plan;
+  size_t off = 0;
+  auto place = [&](size_t n) {
+    const size_t at = off;
+    off += (4 * n + 255) & ~size_t(255);
+    return at;
+  };
+  c->o_list = place(q.list.size());
+  c->o_dep_off = place(q.dep_off.size());
+  c->o_deps = place(q.deps.size() + 1);
+  c->o_segflags = place(q.segflags.size());
+  c->o_rows = place(q.rows.size() + 1);
+  c->bytes = off;
+  if (hipHostMalloc(&c->pinned, off, hipHostMallocDefault) != hipSuccess || hipMalloc(&c->dev, off) != hipSuccess) {
+    (void)hipGetLastError();
+    Scratch::free_l0(c);
+    fail(DWT_GPU_EMEMORY, "allocating a level-0 plan of %zu bytes failed", off);
+  }
+  char* h = static_cast<char*>(c->pinned);
+  auto put = [&](size_t at, const std::vector<int>& v) {
+    if (!v.empty()) std::memcpy(h + at, v.data(), 4 * v.size());
+  };
+  put(c->o_list, q.list);
+  put(c->o_dep_off, q.dep_off);
+  put(c->o_deps, q.deps);
+  put(c->o_segflags, q.segflags);
+  put(c->o_rows, q.rows);
+  HIPCHK(hipMemcpyAsync(c->dev, c->pinned, off, hipMemcpyHostToDevice, st));
+  sc.l0_plans.insert(sc.l0_plans.begin(), c);
+  if (sc.l0_plans.size() > kL0MaxPlans) {
+    HIPCHK(hipStreamSynchronize(st));  // the evicted plan may still be in use
+    Scratch::free_l0(sc.l0_plans.back());
+    sc.l0_plans.pop_back();
+  }
+  return *c;
+}
+
+static L0Dev l0_dev(Scratch& sc, const L0Cached& c, hipStream_t st) {
+  const long long ntiles = (long long)c.plan.nseg * c.plan.ngroups;
+  if (sc.l0_cap < ntiles) {
+    if (sc.l0_words) {
+      HIPCHK(hipStreamSynchronize(st));
+      HIPCHK(hipFree(sc.l0_words));
+      sc.l0_words = nullptr;
+      sc.l0_cap = 0;
+    }
+    const long long cap = std::max<long long>(ntiles, 1 << 14);
+    if (hipMalloc(&sc.l0_words, sizeof(unsigned) * (size_t)(64 + cap)) != hipSuccess) {
+      (void)hipGetLastError();
+      sc.l0_words = nullptr;
+      fail(DWT_GPU_EMEMORY, "allocating level-0 flags failed");
+    }
+    sc.l0_cap = cap;
+    sc.l0_epoch = 0;
+    LAUNCH(launch_fill_u32(sc.l0_words, 0u, 64 + cap, st));  // a kernel, not a memset: capturable
+  }
+  if (++sc.l0_epoch == 0) {  // the epoch wrapped: no flag may hold the new one
+    LAUNCH(launch_fill_u32(sc.l0_words + 64, 0u, sc.l0_cap, st));
+    sc.l0_epoch = 1;
+  }
+  L0Dev d{};
+  const char* base = static_cast<const char*>(c.dev);
+  d.list = reinterpret_cast<const int*>(base + c.o_list);
+  d.dep_off = reinterpret_cast<const int*>(base + c.o_dep_off);
+  d.deps = reinterpret_cast<const int*>(base + c.o_deps);
+  d.segflags = reinterpret_cast<const int*>(base + c.o_segflags);
+  d.flags = sc.l0_words + 64;
+  d.err = sc.l0_words;
+  d.spin_limit = spin_limit_ticks();
+  d.epoch = sc.l0_epoch;
+  for (int x = 0; x < 9; ++x) d.off[x] = c.plan.list_off[x];
+  d.ngroups = c.plan.ngroups;
+  d.nstrips = c.plan.nstrips;
+  return d;
+}
+
+// Segment rows of the in-place level 0 of a W x H image, or 0 when the
+// per-level path runs it: the in-place kernels take the single-shot shapes of
+// 4 and 8 rows; L >= 2 (LL_1 goes to scratch, not into the image); a launch
+// under stream capture would replay one epoch; the allocation-free entry
+// point (sc == null) has no room for plans.
+static int l0_inplace_rows(const Scratch* sc, bool fwd, int W, int H, int L, int nsteps, hipStream_t st) {
+  if (!g_l0_inplace || !sc || L < 2 || W < 2 || H < 2) return 0;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(st, &cs) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  if (cs != hipStreamCaptureStatusNone) return 0;
+  int S = 0, U = 0;
+  tile_get(fwd ? 1 : 0, H, W, nsteps, &S, &U);
+  return U == 0 && (S == 4 || S == 8) ? S : 0;
+}
+
 // ---- the level-0 detail copy, carried by the small levels' launches -----------
 // Level 0 of the in-place forward writes its detail subbands to scratch D
 // (they overlap input that other workgroups have not read yet), and the last
@@ -757,6 +911,12 @@ constexpr int kBlockRiderCap = 480;   // 1024-thread workgroups, two per compute
 
 template <typename T>
 static void copy_alone(const RiderCopy<T>& rc, hipStream_t st) {
+  if (rc.rows) {
+    RiderCopy<T> x = rc;
+    x.blocks = std::min(rc.r1 - rc.r0, kLevelRiderCap);
+    LAUNCH(launch_rider_alone<T>(x, st));
+    return;
+  }
   LAUNCH(launch_copy_rect<T>(rc.src + rc.xa, rc.sp, rc.dst + rc.xa, rc.dp, rc.wa, rc.ha, st);
          launch_copy_rect<T>(rc.src + (long long)rc.ha * rc.sp, rc.sp, rc.dst + (long long)rc.ha * rc.dp,
                              rc.dp, rc.wb, rc.hb, st));
@@ -804,10 +964,12 @@ static FastBufs<T> fast_bufs(Scratch& sc, int W, int H, int L) {
 }
 
 // Forward levels [0, L) of a packed image on the given scratch buffers (unused
-// when the block kernel takes the whole image).  Launches kernels only.
+// when the block kernel takes the whole image).  Launches kernels only.  With
+// a scratch set (sc), level 0 may run in place (l0_inplace_rows); the
+// allocation-free entry point passes none.
 template <typename T>
 static void fast_fwd_on(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, const FastBufs<T>& fb,
-                        hipStream_t st) {
+                        hipStream_t st, Scratch* sc = nullptr) {
   const int jt = tail_start(W, H, L, (int)sizeof(T));
   if (jt == 0) {
     ProfScope ps(4, 0, st);
@@ -827,9 +989,19 @@ static void fast_fwd_on(T* A, long long p, int W, int H, int L, const Coeffs<T>&
     weight.push_back(!mid && bytes <= kCarrierBytes ? 1 : 0);
   }
   if (jt < L) weight.push_back(kBlockCarrierWeight);
-  const RiderCopy<T> rc = detail_copy<T>(D, Dp, A, p, W, H, L);
+  RiderCopy<T> rc = detail_copy<T>(D, Dp, A, p, W, H, L);
+  // in place: only the rows that fell back to D are copied
+  const int S0 = l0_inplace_rows(sc, true, W, H, L, cf.nsteps, st);
+  const L0Cached* l0 = S0 ? &l0_cached(*sc, true, W, H, cf.nsteps, S0, st) : nullptr;
+  int ncopy = H;
+  if (l0) {
+    rc.rows = reinterpret_cast<const int*>(static_cast<const char*>(l0->dev) + l0->o_rows);
+    ncopy = (int)l0->plan.rows.size();
+    rc.r0 = 0;
+    rc.r1 = ncopy;
+  }
   std::vector<int2> rows;
-  const bool carried = split_rows(H, weight, rows);
+  const bool carried = split_rows(ncopy, weight, rows);
   for (int j = 0; j < jt; ++j) {
     const int Wj = ceil_div_pow2(W, j), Hj = ceil_div_pow2(H, j);
     const T* src = j == 0 ? A : P[j % 2];
@@ -861,7 +1033,13 @@ static void fast_fwd_on(T* A, long long p, int W, int H, int L, const Coeffs<T>&
     // duration without the events' dispatch overhead)
     const int reps = j == 0 ? prof_repeat() : 1;
     ProfScope ps(0, j, st, reps);
-    if (j == 0) {
+    if (j == 0 && l0) {
+      // in place (a profiling repeat transforms the image again: timing only)
+      for (int r = 0; r < reps; ++r) {
+        const L0Dev ip = l0_dev(*sc, *l0, st);
+        LAUNCH(launch_fwd_l0<T>(A, p, W, H, ll, llp, D, Dp, cf, ip, S0, l0->plan.grid, st));
+      }
+    } else if (j == 0) {
       for (int r = 0; r < reps; ++r) LAUNCH(launch_fwd_level<T>(src, sp, Wj, Hj, ll, llp, det, dp, cf, st));
     } else {
       const RiderCopy<T> part = carried ? rider_part(rc, rows[j - 1], kLevelRiderCap) : RiderCopy<T>{};
@@ -876,7 +1054,7 @@ static void fast_fwd_on(T* A, long long p, int W, int H, int L, const Coeffs<T>&
     LAUNCH(launch_block_fwd<T>(P[jt % 2], Pp[jt % 2], A, p, ceil_div_pow2(W, jt), ceil_div_pow2(H, jt), L - jt,
                                cf, st, &part));
   }
-  if (!carried) {
+  if (!carried && ncopy > 0) {
     ProfScope ps(2, 0, st);
     copy_alone(rc, st);
   }
@@ -890,7 +1068,7 @@ static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
     return;
   }
   const FastBufs<T> fb = tail_start(W, H, L, (int)sizeof(T)) > 0 ? fast_bufs<T>(sc, W, H, L) : FastBufs<T>{};
-  fast_fwd_on<T>(A, p, W, H, L, cf, fb, st);
+  fast_fwd_on<T>(A, p, W, H, L, cf, fb, st, &sc);
 }
 
 // Inverse of fast_fwd: the block kernel does levels L .. jt+1 (writing LL_jt
@@ -899,7 +1077,7 @@ static void fast_fwd(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
 // A into D, carried by the launches before that last one.
 template <typename T>
 static void fast_inv_on(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf, const FastBufs<T>& fb,
-                        hipStream_t st) {
+                        hipStream_t st, Scratch* sc = nullptr) {
   const int jt = tail_start(W, H, L, (int)sizeof(T));
   if (jt == 0) {
     ProfScope ps(5, 0, st);
@@ -919,11 +1097,21 @@ static void fast_inv_on(T* A, long long p, int W, int H, int L, const Coeffs<T>&
     const long long bytes = (long long)ceil_div_pow2(W, j - 1) * ceil_div_pow2(H, j - 1) * (long long)sizeof(T);
     weight.push_back(!mid && bytes <= kCarrierBytes ? 1 : 0);
   }
-  const RiderCopy<T> rc = detail_copy<T>(A, p, D, Dp, W, H, L);
+  RiderCopy<T> rc = detail_copy<T>(A, p, D, Dp, W, H, L);
+  // in place: only the preserved bands are copied
+  const int S0 = l0_inplace_rows(sc, false, W, H, L, cf.nsteps, st);
+  const L0Cached* l0 = S0 ? &l0_cached(*sc, false, W, H, cf.nsteps, S0, st) : nullptr;
+  int ncopy = H;
+  if (l0) {
+    rc.rows = reinterpret_cast<const int*>(static_cast<const char*>(l0->dev) + l0->o_rows);
+    ncopy = (int)l0->plan.rows.size();
+    rc.r0 = 0;
+    rc.r1 = ncopy;
+  }
   std::vector<int2> rows;
-  const bool carried = split_rows(H, weight, rows);
+  const bool carried = split_rows(ncopy, weight, rows);
   size_t carrier = 0;
-  if (!carried) {
+  if (!carried && ncopy > 0) {
     ProfScope ps(2, 0, st);
     copy_alone(rc, st);
   }
@@ -966,6 +1154,9 @@ static void fast_inv_on(T* A, long long p, int W, int H, int L, const Coeffs<T>&
       const RiderCopy<T> part = carried ? rider_part(rc, rows[carrier], kLevelRiderCap) : RiderCopy<T>{};
       ++carrier;
       LAUNCH(launch_inv_level<T>(ll, llp, det, dp, dst, dstp, Wf, Hf, cf, st, &part));
+    } else if (l0) {
+      const L0Dev ip = l0_dev(*sc, *l0, st);
+      LAUNCH(launch_inv_l0<T>(ll, llp, A, p, D, Dp, W, H, cf, ip, S0, l0->plan.grid, st));
     } else {
       LAUNCH(launch_inv_level<T>(ll, llp, det, dp, dst, dstp, Wf, Hf, cf, st));
     }
@@ -980,7 +1171,7 @@ static void fast_inv(T* A, long long p, int W, int H, int L, const Coeffs<T>& cf
     return;
   }
   const FastBufs<T> fb = tail_start(W, H, L, (int)sizeof(T)) > 0 ? fast_bufs<T>(sc, W, H, L) : FastBufs<T>{};
-  fast_inv_on<T>(A, p, W, H, L, cf, fb, st);
+  fast_inv_on<T>(A, p, W, H, L, cf, fb, st, &sc);
 }
 
 // ---- 2D driver on device-accessible memory -------------------------------------
@@ -1185,6 +1376,7 @@ static void run2d(bool fwd, void* ptr, int sx, int sy, int Wo, int Ho, int Wi, i
     if (sync) {
       HIPCHK(hipStreamSynchronize(st));
       if (pyr_failed(scratch_for(st), st)) fail(DWT_GPU_EHIP, "pyramid dependency wait timed out");
+    if (l0_failed(scratch_for(st), st)) fail(DWT_GPU_EHIP, "in-place level-0 dependency wait timed out");
     }
     return;
   }
@@ -2036,6 +2228,45 @@ int dwt_gpu_col_segment_rows(int forward, int n, int nsteps, int seg, int* reads
   std::copy(w.begin(), w.end(), writes);
   *nreads = (int)r.size();
   *nwrites = (int)w.size();
+  return DWT_GPU_OK;
+}
+
+int dwt_gpu_level0_set(int inplace, int band) {
+  if (inplace < 0 || inplace > 1 || band < 0 || band > 1 << 20) {
+    g_err = "dwt_gpu_level0_set: bad argument";
+    return DWT_GPU_EINVAL;
+  }
+  g_l0_inplace = inplace;
+  if (band > 0) g_l0_band = band;
+  return DWT_GPU_OK;
+}
+
+int dwt_gpu_level0_plan(int forward, int width, int height, int wavelet, int seg_rows, int* info, int* order,
+                        int* segflags, int* list_off, int* list, int* dep_off, int* deps, int* rows) {
+  if (!info || width < 2 || height < 2 || (wavelet != DWT_GPU_CDF97 && wavelet != DWT_GPU_CDF53) ||
+      (seg_rows != 4 && seg_rows != 8)) {
+    g_err = "dwt_gpu_level0_plan: bad argument";
+    return DWT_GPU_EINVAL;
+  }
+  const L0Plan P = build_l0_plan(forward != 0, width, height, wavelet == DWT_GPU_CDF97 ? 4 : 2, seg_rows);
+  info[0] = P.nseg;
+  info[1] = P.ngroups;
+  info[2] = P.nstrips;
+  info[3] = P.nseg * P.ngroups;
+  info[4] = (int)P.deps.size();
+  info[5] = (int)P.rows.size();
+  info[6] = P.grid;
+  info[7] = P.band;
+  auto out = [](const std::vector<int>& v, int* o) {
+    if (o) std::copy(v.begin(), v.end(), o);
+  };
+  out(P.order, order);
+  out(P.segflags, segflags);
+  out(P.list_off, list_off);
+  out(P.list, list);
+  out(P.dep_off, dep_off);
+  out(P.deps, deps);
+  out(P.rows, rows);
   return DWT_GPU_OK;
 }
 

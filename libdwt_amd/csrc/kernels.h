@@ -37,7 +37,52 @@ struct RiderCopy {
   int r0, r1;
   int blocks;
   int vec;
+  const int* rows;  // device list of image rows (entry i is row rows[i]); null: row i
 };
+
+// ---- in-place level 0 (inplace.hip) -------------------------------------------
+// Plan of one level-0 launch on a W x H image (tiles of S-row segments, four
+// strips per workgroup): the segment order, per segment the parts that fall
+// back to scratch (forward: bit 0 HL rows, bit 1 LH + HH rows; inverse: bit 0
+// the band is preserved in D), the tiles of each XCD in order (list[off[x] ..
+// off[x+1]) for workgroups b = 8 k + x), every tile's dependencies (tiles
+// whose reads must be done before it stores; CSR), and the image rows the
+// riders copy (forward: D -> image after level 0; inverse: image -> D
+// before it).  grid: workgroups of the launch.
+struct L0Plan {
+  bool fwd = true;
+  int W = 0, H = 0, nsteps = 4, S = 8, band = 8;
+  int nstrips = 0, ngroups = 0, nseg = 0, grid = 0;
+  std::vector<int> order, segflags, list, list_off, dep_off, deps, rows;
+};
+L0Plan build_l0_plan(bool fwd, int W, int H, int nsteps, int S);
+extern int g_l0_band;
+// Device view of a plan: tables in device memory, the flags (one word per
+// tile) and error word of the scratch set, this launch's epoch.
+struct L0Dev {
+  const int* list;
+  const int* dep_off;
+  const int* deps;
+  const int* segflags;
+  unsigned* flags;
+  unsigned* err;
+  unsigned long long spin_limit;
+  unsigned epoch;
+  int off[9];
+  int ngroups, nstrips;
+};
+// Forward level 0 of the W x H image A (pitch ap) in place: LL to ll, details
+// to A (or D, pitch dp, for the parts that fall back).  Inverse: LL from ll,
+// details from A (D for preserved bands), output to A.
+template <typename T>
+void launch_fwd_l0(T* A, long long ap, int W, int H, T* ll, long long llp, T* D, long long dp, const Coeffs<T>& cf,
+                   const L0Dev& ip, int S, int grid, hipStream_t st);
+template <typename T>
+void launch_inv_l0(const T* ll, long long llp, T* A, long long ap, const T* D, long long dp, int W, int H,
+                   const Coeffs<T>& cf, const L0Dev& ip, int S, int grid, hipStream_t st);
+// A rider copy as a launch of its own (rc.blocks workgroups).
+template <typename T>
+void launch_rider_alone(const RiderCopy<T>& rc, hipStream_t st);
 
 // ---- packed fast path (tile_kernels.hip) ------------------------------------
 // Forward level on a packed W x H region `src` (row pitch sp elements):

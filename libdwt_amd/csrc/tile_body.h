@@ -118,7 +118,8 @@ template <typename T>
 __device__ __forceinline__ void rider_copy(const RiderCopy<T>& rc, int k, int nwg) {
   using V = typename std::conditional<sizeof(T) == 4, Vec16f, Vec16d>::type;
   constexpr int PER = 16 / sizeof(T);  // elements of one 16-byte vector
-  for (int y = rc.r0 + k; y < rc.r1; y += nwg) {
+  for (int i = rc.r0 + k; i < rc.r1; i += nwg) {
+    const int y = rc.rows ? rc.rows[i] : i;
     const int x0 = y < rc.ha ? rc.xa : 0, w = y < rc.ha ? rc.wa : rc.wb;
     const Gbl<T>* s = gload_ptr(rc.src) + (long long)y * rc.sp + x0;
     Gbl<T>* d = gstore_ptr(rc.dst) + (long long)y * rc.dp + x0;
