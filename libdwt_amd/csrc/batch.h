@@ -14,7 +14,7 @@ constexpr int kStoreF32 = 0, kStoreF16 = 1, kStoreBF16 = 2;
 // its LL comes from the coefficients.
 constexpr int kItemStorageLL = 1;
 // Item flag: the FP32 rows it reads (forward) or writes (inverse) allow
-// two-element vector access (pair-aligned pointer, even pitch).
+// two-element vector access (element-aligned pointer; any pitch).
 constexpr int kItemVec = 2;
 
 // One image at one level of a batched launch.  Forward: src = level input

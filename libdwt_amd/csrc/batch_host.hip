@@ -140,10 +140,10 @@ void BatchTables::build(int wavelet_, bool fwd_, int storage_, const dwt_gpu_ima
           e.op = llbuf(i, j).second;
         }
       }
-      // vector access on the FP32 rows: scratch rows always allow it, the
-      // image's only when pair-aligned
-      const bool img_vec = (reinterpret_cast<uintptr_t>(fwd ? im.src : (const void*)im.dst) & 7) == 0 &&
-                           ip % 2 == 0;
+      // vector access on the FP32 rows: a pair moves as one element-aligned
+      // access (GPair), so every row of an element-aligned image allows it,
+      // odd widths included
+      const bool img_vec = (reinterpret_cast<uintptr_t>(fwd ? im.src : (const void*)im.dst) & 3) == 0;
       if (j > 0 || img_vec) e.flags |= kItemVec;
       e.nstrips = (e.W + core - 1) / core;
       e.ngroups = (e.nstrips + 3) / 4;

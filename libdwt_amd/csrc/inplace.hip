@@ -654,8 +654,9 @@ inv_l0_kernel(const T* __restrict__ ll, long long llp, T* A, long long ap, const
 // ---- host: launchers --------------------------------------------------------------
 
 template <typename T>
-static bool pair_ok(const void* p, long long pitch) {
-  return (reinterpret_cast<uintptr_t>(p) % (2 * sizeof(T))) == 0 && (pitch % 2) == 0;
+static bool pair_ok(const void* p, long long) {
+  // pairs move as one element-aligned access (GPair): any row pitch will do
+  return (reinterpret_cast<uintptr_t>(p) % sizeof(T)) == 0;
 }
 
 template <typename T, int NS, bool VEC>

@@ -139,6 +139,14 @@ __device__ __forceinline__ void rider_copy(const RiderCopy<T>& rc, int k, int nw
 template <typename T> struct Vec2;
 template <> struct Vec2<float> { typedef float type __attribute__((ext_vector_type(2))); };
 template <> struct Vec2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+// The same pair for global memory, aligned to one element only.  gfx950 does
+// multi-dword global loads and stores at dword alignment (the compiler emits
+// global_load_dwordx2 for a pointer known to be only 4-byte aligned), so a
+// pair that starts at an odd element -- every other row of an image with an
+// odd pitch -- still moves as one access instead of two.
+template <typename T> struct GPair;
+template <> struct GPair<float> { typedef float type __attribute__((ext_vector_type(2), aligned(4))); };
+template <> struct GPair<double> { typedef double type __attribute__((ext_vector_type(2), aligned(8))); };
 
 // Forward lifting of one row: lane holds (even, odd) samples; odd samples are
 // updated first.  Result: (L, H) coefficient of the lane's pair.
@@ -520,7 +528,7 @@ struct GlobalRows {  // packed rows in global memory
   int H;
   __device__ __forceinline__ void operator()(int Y, int Xe, int xe, int xo, bool interior, T& e,
                                              T& o) const {
-    using V2 = typename Vec2<T>::type;
+    using V2 = typename GPair<T>::type;
     const Gbl<T>* rp = gload_ptr(p) + (long long)Y * pitch;
     if (VEC && interior) {
       const V2 v = *(const Gbl<V2>*)(rp + Xe);
@@ -593,7 +601,7 @@ struct GlobalOut {  // two adjacent output samples, global
   T* __restrict__ p;
   long long pitch;
   __device__ __forceinline__ void operator()(int r, int Xe, T e, T o, bool st_e, bool st_o) const {
-    using V2 = typename Vec2<T>::type;
+    using V2 = typename GPair<T>::type;
     T* rp = p + (long long)r * pitch + Xe;
 #if DWT_STORE_POLICY == 0
     if (VEC && st_o) {

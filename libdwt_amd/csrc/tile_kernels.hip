@@ -172,8 +172,11 @@ int tile_get(int fwd, int H, int W, int nsteps, int* S, int* U) {
 }
 
 template <typename T>
-static bool aligned_pair(const void* p, long long pitch) {
-  return (reinterpret_cast<uintptr_t>(p) % (2 * sizeof(T))) == 0 && (pitch % 2) == 0;
+static bool aligned_pair(const void* p, long long) {
+  // a sample pair moves as one element-aligned access (GPair, tile_body.h),
+  // so rows of an odd pitch, half of which start at an odd element, take the
+  // vector path too; only a pointer that is not element-aligned does not
+  return (reinterpret_cast<uintptr_t>(p) % sizeof(T)) == 0;
 }
 
 // Instantiation for a tile shape: rolling prefetch (U = 4 / 8, any S) or
