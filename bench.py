@@ -328,6 +328,10 @@ def measure_batch(wv, steps, warm, torch, libdwt_amd, workloads, storages=("f32"
                 n, t = libdwt_amd.profile_read(kind, lev)
                 if n:
                     kern[f"{name}{lev}"] = round(1000 * t / n, 2)
+        for kind, name in ((12, "fwd_tail"), (13, "inv_tail")):  # every image's small levels
+            n, t = libdwt_amd.profile_read(kind, -1)
+            if n:
+                kern[name] = round(1000 * t / n, 2)
         libdwt_amd.profile_reset()
         # errors on the first images
         cerr, rerr, psnr = 0.0, 0.0, float("inf")

@@ -354,7 +354,8 @@ int dwt_gpu_verify_2d(int wavelet, int direction, const void *input, const void 
  * kernel, 2 level-0 detail copy, 4 forward block kernel, 5 inverse block
  * kernel, 6 forward pyramid, 7 inverse pyramid (levels after level 0),
  * 8 / 9 forward / inverse batched level launch, 10 / 11 forward / inverse
- * fused middle levels (dwt_gpu_mid_set);
+ * fused middle levels (dwt_gpu_mid_set), 12 / 13 forward / inverse batched
+ * tail (dwt_gpu_batch_tail_set; level -1);
  * level: the level index, the first level it does for a block kernel or a
  * pyramid (-1 = all).
  * on: 0 off, 1 on, k >= 2 on with every level-0 forward kernel of the packed
@@ -500,6 +501,12 @@ int dwt_gpu_batch_plan_destroy(void *plan);
 /* Segment rows (8, 16 or 32; 0: default 8) of the batched launches of one
  * level, for plans created afterwards. */
 int dwt_gpu_batch_rows_set(int level, int seg_rows);
+/* Batched tail: from the first level whose region is at most max_region_bytes
+ * of FP32 samples (and at most 512 samples a side), every remaining level of
+ * every image runs in one launch, one workgroup per image, in LDS
+ * (bit-identical to the per-level launches).  0 turns it off; at most 131072;
+ * default 65536.  For plans created afterwards. */
+int dwt_gpu_batch_tail_set(long long max_region_bytes);
 
 /* ---- serving loop -----------------------------------------------------------
  * A server takes transform requests (one image each, buffers resident in

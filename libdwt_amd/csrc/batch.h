@@ -37,6 +37,23 @@ struct BatchItem {
   int flags, reserved;
 };
 
+// The small levels of one image (levels `level` .. levels - 1, a W x H region
+// of at most kBatchTailMax bytes), all in one workgroup's LDS.  Forward: src =
+// the level's input (FP32: the image, or LL scratch), dst = the coefficients
+// (storage type); inverse: src = the coefficients, dst = the FP32 output (the
+// image, or LL scratch for the next finer level).  Pitches in elements.
+struct BatchTail {
+  const void* src;
+  long long sp;
+  void* dst;
+  long long dp;
+  float* scale;
+  int W, H, level, nlev, levels, reserved;
+};
+// Largest tail region (bytes of FP32 samples); its LDS buffer (odd pitch)
+// plus the scales stays within one workgroup's 160 KiB.
+constexpr long long kBatchTailMax = 128 * 1024;
+
 // One image of a batch for the max |x| pass and the scale computation.
 struct BatchImage {
   const void* src;
@@ -44,12 +61,20 @@ struct BatchImage {
   float* scale;
   int W, H, levels, block0;
 };
-// The max |x| pass: a block takes maxabs_rows(W) whole rows of one image
-// (about 16 pixels per thread per pass; one row, in passes of kMaxAbsPixels,
-// when a row is longer), so no index needs a division.
+// The max |x| pass.  A contiguous image (pitch == width) is one flat range,
+// a block taking kMaxAbsFlat consecutive pixels (eight 16-byte loads per
+// thread); otherwise a block takes maxabs_rows(W) whole rows (about 16 pixels
+// per thread per pass; one row, in passes of kMaxAbsPixels, when a row is
+// longer), so no index needs a division.
 constexpr int kMaxAbsPixels = 256 * 16;
+constexpr int kMaxAbsFlat = 256 * 32;
 __host__ __device__ inline int maxabs_rows(int W) {
   return W >= kMaxAbsPixels ? 1 : kMaxAbsPixels / (W > 0 ? W : 1);
+}
+__host__ __device__ inline long long maxabs_blocks(int W, int H, long long sp) {
+  if (sp == W) return ((long long)W * H + kMaxAbsFlat - 1) / kMaxAbsFlat;
+  const int rows = maxabs_rows(W);
+  return (H + rows - 1) / rows;
 }
 
 void launch_batch_level(bool fwd, int storage, int S, int nblocks, const BatchItem* items,
@@ -61,5 +86,12 @@ void launch_batch_maxabs(const BatchImage* imgs, const int* block_img, int nbloc
 void launch_batch_scales(const BatchImage* imgs, int n, const unsigned* amax, const float* gain,
                          hipStream_t st);
 void analysis_gains(const Coeffs<double>& cf, double* gl, double* gh);
+// The tails of n images, one workgroup each (lds: dynamic LDS bytes, enough
+// for the largest region and its scales).
+void launch_batch_tail(bool fwd, int storage, int n, size_t lds, const BatchTail* tails,
+                       const Coeffs<float>& cf, hipStream_t st);
+// LDS bytes of a tail region of W x H samples: odd-pitch buffer, then the
+// scales and their inverses.
+size_t batch_tail_lds(int W, int H);
 
 }  // namespace dwt

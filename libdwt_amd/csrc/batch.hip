@@ -22,7 +22,7 @@
 // overflow, and the subband's largest coefficient uses the top of the FP16
 // exponent range.  Scales: 3 per level (HL, LH, HH) then one for the final
 // LL, in a float array per image that the inverse reads back.
-#include "tile_body.h"
+#include "block_body.h"
 #include "kernels.h"
 #include "batch.h"
 
@@ -194,6 +194,33 @@ batch_maxabs_kernel(const BatchImage* __restrict__ imgs, const int* __restrict__
   __shared__ unsigned red[4];
   const int i = block_img[blockIdx.x];
   const BatchImage& im = imgs[i];
+  if (im.sp == im.W) {
+    // contiguous image: pixels [c0, c0 + kMaxAbsFlat) of the flat range,
+    // element-aligned 16-byte loads (any base alignment), all issued first
+    using F4 = float __attribute__((ext_vector_type(4), aligned(4)));
+    constexpr int K = kMaxAbsFlat / 256 / 4;
+    const long long n = (long long)im.W * im.H;
+    const long long c0 = (long long)(blockIdx.x - im.block0) * kMaxAbsFlat;
+    const int cnt = (int)min((long long)kMaxAbsFlat, n - c0);
+    const Gbl<float>* src = gload_ptr(static_cast<const float*>(im.src)) + c0;
+    float m = 0.0f;
+    F4 v[K];
+#pragma unroll
+    for (int u = 0; u < K; ++u) {
+      const int e = 4 * (threadIdx.x + 256 * u);
+      v[u] = e + 3 < cnt ? *reinterpret_cast<const Gbl<F4>*>(src + e) : F4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+#pragma unroll
+    for (int u = 0; u < K; ++u)
+      m = fmaxf(m, fmaxf(fmaxf(fabsf(v[u].x), fabsf(v[u].y)), fmaxf(fabsf(v[u].z), fabsf(v[u].w))));
+    // the last block's pixels past its whole groups of four
+    for (int e = (cnt & ~3) + threadIdx.x; e < cnt; e += 256) m = fmaxf(m, fabsf(src[e]));
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = __float_as_uint(m);
+    __syncthreads();
+    if (threadIdx.x == 0) amax[blockIdx.x] = max(max(red[0], red[1]), max(red[2], red[3]));
+    return;
+  }
   const int W = im.W, R = maxabs_rows(W);
   const int y0 = (blockIdx.x - im.block0) * R;
   const int n = min(R, im.H - y0) * W;  // pixels of this block's rows
@@ -270,7 +297,7 @@ batch_scales_kernel(const BatchImage* __restrict__ imgs, int n, const unsigned* 
   const int i = blockIdx.x;
   if (i >= n) return;
   const BatchImage& im = imgs[i];
-  const int nb = (im.H + maxabs_rows(im.W) - 1) / maxabs_rows(im.W);
+  const int nb = (int)maxabs_blocks(im.W, im.H, im.sp);
   unsigned mb = 0;
   for (int b = threadIdx.x; b < nb; b += 256) mb = max(mb, amax[im.block0 + b]);
   for (int off = 32; off > 0; off >>= 1) mb = max(mb, (unsigned)__shfl_xor((int)mb, off));
@@ -291,7 +318,202 @@ batch_scales_kernel(const BatchImage* __restrict__ imgs, int n, const unsigned* 
   }
 }
 
+// ---- the small levels of every image: one workgroup per image -----------------------
+// A level of a small image is a few tiles, and a launch per level for the
+// whole batch is latency-bound (about 8 us each, whatever the batch).  Here
+// the levels from the first whose region fits kBatchTailMax run as one
+// launch, image b in workgroup b, with the block kernels' line-parallel
+// lifting in LDS (block_body.h), whose arithmetic is the level kernels' bit
+// for bit.  16-bit coefficients are converted on the way out (forward) or in
+// (inverse) with the scale of the subband each position belongs to, as the
+// level kernels' Det16 / Store16 / Load16 do.
+
+// Level (0 .. nlev - 1, relative to the region) whose detail half holds
+// position x of a region side of n samples after nlev levels; nlev for the
+// final LL.
+__device__ __forceinline__ int tail_level(int x, int n, int nlev) {
+  for (int l = 0; l < nlev; ++l)
+    if (x >= ceil_div_pow2(n, l + 1)) return l;
+  return nlev;
+}
+// Index into the image's scales of position (x, y) of the region: 3 per level
+// (HL, LH, HH) from level t.level on, then the final LL's at 3 * levels.
+__device__ __forceinline__ int tail_scale_index(int lx, int ly, const BatchTail& t) {
+  const int l = min(lx, ly);
+  if (l == t.nlev) return 3 * t.levels;
+  return 3 * (t.level + l) + (lx == l ? (ly == l ? kHH : kHL) : kLH);
+}
+
+// The region between the coefficients (16-bit, global) and the FP32 LDS
+// buffer C (pitch pc); S holds the scales (forward: multiply) or their
+// inverses (inverse: multiply), indexed as tail_scale_index.  As block_copy:
+// wavefront w takes rows w, w + 16, ..., lanes consecutive samples in NCH
+// chunks of 64, and a round issues all of its loads before the first is
+// used (one memory round trip per round, not per row).
+template <int SK, bool TO_LDS, int NCH>
+__device__ __forceinline__ void tail_copy16_rows(Lds<float>* C, int pc, const BatchTail& t, const Lds<float>* S) {
+  constexpr int RPR = kBlockRound / NCH;  // rows per round
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const long long gp = TO_LDS ? t.sp : t.dp;
+  int lx[NCH];
+  bool in[NCH];
+#pragma unroll
+  for (int c = 0; c < NCH; ++c) {
+    in[c] = lane + c * kWave < t.W;
+    lx[c] = tail_level(lane + c * kWave, t.W, t.nlev);
+  }
+  for (int y0 = wave; y0 < t.H; y0 += RPR * kBlockWaves) {
+    float v[RPR][NCH];
+#pragma unroll
+    for (int r = 0; r < RPR; ++r) {
+      const int y = y0 + r * kBlockWaves;
+      if (y < t.H) {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+          const int x = lane + c * kWave;
+          if constexpr (TO_LDS)
+            v[r][c] = in[c] ? from_storage<SK>(gload_ptr(static_cast<const unsigned short*>(t.src))[y * gp + x]) : 0.0f;
+          else
+            v[r][c] = in[c] ? C[y * pc + x] : 0.0f;
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RPR; ++r) {
+      const int y = y0 + r * kBlockWaves;
+      if (y < t.H) {
+        const int ly = tail_level(y, t.H, t.nlev);
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) {
+          const int x = lane + c * kWave;
+          if (!in[c]) continue;
+          const float s = S[tail_scale_index(lx[c], ly, t)];
+          if constexpr (TO_LDS)
+            C[y * pc + x] = v[r][c] * s;
+          else
+            gstore_ptr(static_cast<unsigned short*>(t.dst))[y * gp + x] = to_storage<SK>(v[r][c] * s);
+        }
+      }
+    }
+  }
+}
+template <int SK, bool TO_LDS>
+__device__ __forceinline__ void tail_copy16(Lds<float>* C, int pc, const BatchTail& t, const Lds<float>* S) {
+  if (t.W <= kWave)
+    tail_copy16_rows<SK, TO_LDS, 1>(C, pc, t, S);
+  else if (t.W <= 2 * kWave)
+    tail_copy16_rows<SK, TO_LDS, 2>(C, pc, t, S);
+  else if (t.W <= 4 * kWave)
+    tail_copy16_rows<SK, TO_LDS, 4>(C, pc, t, S);
+  else
+    tail_copy16_rows<SK, TO_LDS, 8>(C, pc, t, S);
+}
+
+template <int NS, int SK>
+__global__ void __launch_bounds__(kBlockThreads)
+batch_tail_fwd_kernel(const BatchTail* __restrict__ tails, Coeffs<float> cf) {
+  extern __shared__ __align__(16) unsigned char blk_smem[];
+  const BatchTail t = tails[blockIdx.x];
+  Lds<float>* C = (Lds<float>*)blk_smem;  // generic -> LDS address-space cast
+  const int pc = block_pitch(t.W);
+  Lds<float>* S = C + pc * t.H;
+  if constexpr (SK != kStoreF32)
+    for (int k = threadIdx.x; k <= 3 * t.levels; k += kBlockThreads) S[k] = gload_ptr(t.scale)[k];
+  block_copy<true>(C, pc, const_cast<float*>(static_cast<const float*>(t.src)), t.sp, t.W, t.H);
+  __syncthreads();
+  for (int lev = 0; lev < t.nlev; ++lev) {
+    const int W = ceil_div_pow2(t.W, lev), H = ceil_div_pow2(t.H, lev);
+    block_pass_any<true, float, NS>(C, pc, 1, H, W, cf);  // rows
+    __syncthreads();
+    block_pass_any<true, float, NS>(C, 1, pc, W, H, cf);  // columns
+    __syncthreads();
+  }
+  if constexpr (SK == kStoreF32)
+    block_copy<false>(C, pc, static_cast<float*>(t.dst), t.dp, t.W, t.H);
+  else
+    tail_copy16<SK, false>(C, pc, t, S);
+}
+
+template <int NS, int SK>
+__global__ void __launch_bounds__(kBlockThreads)
+batch_tail_inv_kernel(const BatchTail* __restrict__ tails, Coeffs<float> cf) {
+  extern __shared__ __align__(16) unsigned char blk_smem[];
+  const BatchTail t = tails[blockIdx.x];
+  Lds<float>* C = (Lds<float>*)blk_smem;  // generic -> LDS address-space cast
+  const int pc = block_pitch(t.W);
+  Lds<float>* S = C + pc * t.H;
+  if constexpr (SK == kStoreF32) {
+    block_copy<true>(C, pc, const_cast<float*>(static_cast<const float*>(t.src)), t.sp, t.W, t.H);
+  } else {
+    for (int k = threadIdx.x; k <= 3 * t.levels; k += kBlockThreads) S[k] = 1.0f / gload_ptr(t.scale)[k];
+    __syncthreads();
+    tail_copy16<SK, true>(C, pc, t, S);
+  }
+  __syncthreads();
+  for (int lev = t.nlev - 1; lev >= 0; --lev) {
+    const int W = ceil_div_pow2(t.W, lev), H = ceil_div_pow2(t.H, lev);
+    block_pass_any<false, float, NS>(C, pc, 1, H, W, cf);  // rows
+    __syncthreads();
+    block_pass_any<false, float, NS>(C, 1, pc, W, H, cf);  // columns
+    __syncthreads();
+  }
+  block_copy<false>(C, pc, static_cast<float*>(t.dst), t.dp, t.W, t.H);
+}
+
 // ---- host launchers --------------------------------------------------------------------
+size_t batch_tail_lds(int W, int H) {
+  return sizeof(float) * ((size_t)block_pitch(W) * H + 3 * 32 + 1);
+}
+
+
+template <int NS, int SK>
+static void tail_go(bool fwd, int n, size_t lds, const BatchTail* tails, const Coeffs<float>& cf,
+                    hipStream_t st) {
+  if (fwd)
+    hipLaunchKernelGGL((batch_tail_fwd_kernel<NS, SK>), dim3(n), dim3(kBlockThreads), lds, st, tails, cf);
+  else
+    hipLaunchKernelGGL((batch_tail_inv_kernel<NS, SK>), dim3(n), dim3(kBlockThreads), lds, st, tails, cf);
+}
+
+void launch_batch_tail(bool fwd, int storage, int n, size_t lds, const BatchTail* tails,
+                       const Coeffs<float>& cf, hipStream_t st) {
+  if (n <= 0) return;
+  if (cf.nsteps == 4) {
+    if (storage == kStoreF16) tail_go<4, kStoreF16>(fwd, n, lds, tails, cf, st);
+    else if (storage == kStoreBF16) tail_go<4, kStoreBF16>(fwd, n, lds, tails, cf, st);
+    else tail_go<4, kStoreF32>(fwd, n, lds, tails, cf, st);
+  } else {
+    if (storage == kStoreF16) tail_go<2, kStoreF16>(fwd, n, lds, tails, cf, st);
+    else if (storage == kStoreBF16) tail_go<2, kStoreBF16>(fwd, n, lds, tails, cf, st);
+    else tail_go<2, kStoreF32>(fwd, n, lds, tails, cf, st);
+  }
+}
+
+hipError_t configure_batch_tail() {
+  const void* fns[] = {
+      reinterpret_cast<const void*>(&batch_tail_fwd_kernel<4, kStoreF32>),
+      reinterpret_cast<const void*>(&batch_tail_fwd_kernel<4, kStoreF16>),
+      reinterpret_cast<const void*>(&batch_tail_fwd_kernel<4, kStoreBF16>),
+      reinterpret_cast<const void*>(&batch_tail_fwd_kernel<2, kStoreF32>),
+      reinterpret_cast<const void*>(&batch_tail_fwd_kernel<2, kStoreF16>),
+      reinterpret_cast<const void*>(&batch_tail_fwd_kernel<2, kStoreBF16>),
+      reinterpret_cast<const void*>(&batch_tail_inv_kernel<4, kStoreF32>),
+      reinterpret_cast<const void*>(&batch_tail_inv_kernel<4, kStoreF16>),
+      reinterpret_cast<const void*>(&batch_tail_inv_kernel<4, kStoreBF16>),
+      reinterpret_cast<const void*>(&batch_tail_inv_kernel<2, kStoreF32>),
+      reinterpret_cast<const void*>(&batch_tail_inv_kernel<2, kStoreF16>),
+      reinterpret_cast<const void*>(&batch_tail_inv_kernel<2, kStoreBF16>)};
+  for (const void* f : fns) {
+    const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kBlockLdsMax);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      return e;
+    }
+  }
+  return hipSuccess;
+}
+
 template <int NS, int SK>
 static void batch_go(bool fwd, int S, int nblocks, const BatchItem* items, const int* block_item,
                      const Coeffs<float>& cf, hipStream_t st) {

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -10,6 +11,7 @@
 #include <vector>
 
 #include "batch.h"
+#include "block_body.h"
 #include "batch_tables.h"
 #include "host_common.h"
 #include "kernels.h"
@@ -25,6 +27,21 @@ namespace dwt {
 // device; executing it only launches kernels, so it can be captured in a HIP
 // graph and replayed.  The serving loop builds tables per admitted batch.
 static int g_batch_rows[32] = {};  // segment rows per level (0: 8)
+static long long g_batch_tail_bytes = 64 * 1024;  // tail regions up to this size (0: no tail)
+// Bumped by every change of the two settings above: a cached plan built under
+// other settings is not reused.
+static std::atomic<int> g_batch_config_gen{0};
+
+int batch_tail_start(int W, int H, int J) {
+  if (g_batch_tail_bytes <= 0) return J;
+  for (int j = 0; j < J; ++j) {
+    const int w = ceil_div_pow2(W, j), h = ceil_div_pow2(H, j);
+    if (4LL * w * h <= g_batch_tail_bytes && w <= kBlockMaxLine && h <= kBlockMaxLine &&
+        batch_tail_lds(w, h) <= (size_t)kBlockLdsMax)
+      return j;
+  }
+  return J;
+}
 
 static size_t storage_bytes(int storage) { return storage == kStoreF32 ? 4 : 2; }
 
@@ -80,6 +97,14 @@ void BatchTables::build(int wavelet_, bool fwd_, int storage_, const dwt_gpu_ima
   items.clear();
   bmap.clear();
   levels.clear();
+  tails.clear();
+  tail_lds = 0;
+  std::vector<int> jt(n);
+  int Lp = 0;  // levels run by per-level launches
+  for (int i = 0; i < n; ++i) {
+    jt[i] = batch_tail_start(images[i].width, images[i].height, images[i].levels);
+    Lp = std::max(Lp, jt[i]);
+  }
   auto llbuf = [&](int i, int j) -> std::pair<float*, long long> {  // LL_j, j >= 1
     const dwt_gpu_image& im = images[i];
     const long long p1 = (long long)pad_pitch(ceil_div_pow2(im.width, 1));
@@ -88,14 +113,14 @@ void BatchTables::build(int wavelet_, bool fwd_, int storage_, const dwt_gpu_ima
     return (j & 1) ? std::make_pair((float*)scratch[i], p1)
                    : std::make_pair((float*)(scratch[i] + b1), p2);
   };
-  for (int step = 0; step < Lmax; ++step) {
-    const int j = fwd ? step : Lmax - 1 - step;
+  for (int step = 0; step < Lp; ++step) {
+    const int j = fwd ? step : Lp - 1 - step;
     BatchLevel lv{j, (int)items.size(), 0, (int)bmap.size(), g_batch_rows[j] ? g_batch_rows[j] : 8};
     int blocks = 0;
     for (int i = 0; i < n; ++i) {
       const dwt_gpu_image& im = images[i];
       const int J = im.levels;
-      if (j >= J) continue;
+      if (j >= jt[i]) continue;  // this level and the ones after it: the tail
       BatchItem e{};
       const long long ip = (fwd ? im.src_pitch : im.dst_pitch) / 4;
       const long long cp = (fwd ? im.dst_pitch : im.src_pitch) / (long long)cs;
@@ -158,17 +183,43 @@ void BatchTables::build(int wavelet_, bool fwd_, int storage_, const dwt_gpu_ima
     lv.nblocks = blocks;
     levels.push_back(lv);
   }
+  for (int i = 0; i < n; ++i) {
+    const dwt_gpu_image& im = images[i];
+    const int j = jt[i], J = im.levels;
+    if (j >= J) continue;
+    const long long ip = (fwd ? im.src_pitch : im.dst_pitch) / 4;
+    const long long cp = (fwd ? im.dst_pitch : im.src_pitch) / (long long)cs;
+    BatchTail t{};
+    t.W = ceil_div_pow2(im.width, j);
+    t.H = ceil_div_pow2(im.height, j);
+    t.level = j;
+    t.nlev = J - j;
+    t.levels = J;
+    t.scale = im.scales;
+    // the FP32 side: the image at level 0, else LL_j in the ping-pong scratch
+    void* fp = j == 0 ? (fwd ? const_cast<void*>(im.src) : im.dst) : (void*)llbuf(i, j).first;
+    const long long fpp = j == 0 ? ip : llbuf(i, j).second;
+    void* coef = fwd ? im.dst : const_cast<void*>(im.src);
+    t.src = fwd ? fp : coef;
+    t.sp = fwd ? fpp : cp;
+    t.dst = fwd ? coef : fp;
+    t.dp = fwd ? cp : fpp;
+    tails.push_back(t);
+    tail_lds = std::max(tail_lds, batch_tail_lds(t.W, t.H));
+  }
   imgs.clear();
   imap.clear();
   gain.assign(96 + 33, 1.0f);
   if (fwd && storage != kStoreF32) {
-    for (int i = 0; i < n; ++i) {
+    // the max |x| pass reads the images last to first, so that the ones it
+    // read last -- still in the 256 MB Infinity Cache -- are the ones the
+    // level-0 launch (image-major, first image first) reads first
+    imgs.resize(n);
+    for (int i = n - 1; i >= 0; --i) {
       const dwt_gpu_image& im = images[i];
-      BatchImage b{im.src, im.src_pitch / 4, im.scales, im.width, im.height, im.levels, (int)imap.size()};
-      const int rows = maxabs_rows(im.width);
-      const long long nb = (im.height + rows - 1) / rows;
+      imgs[i] = BatchImage{im.src, im.src_pitch / 4, im.scales, im.width, im.height, im.levels, (int)imap.size()};
+      const long long nb = maxabs_blocks(im.width, im.height, im.src_pitch / 4);
       imap.insert(imap.end(), (size_t)nb, i);
-      imgs.push_back(b);
     }
     double gl = 0, gh = 0;
     analysis_gains(make_coeffs<double>(wavelet == DWT_GPU_CDF97 ? CDF97 : CDF53, true), &gl, &gh);
@@ -187,7 +238,8 @@ void BatchTables::build(int wavelet_, bool fwd_, int storage_, const dwt_gpu_ima
   off_imap = off_imgs + al(sizeof(BatchImage) * imgs.size());
   off_gain = off_imap + al(sizeof(int) * imap.size());
   off_amax = off_gain + al(sizeof(float) * gain.size());
-  bytes = off_amax + al(sizeof(unsigned) * (imap.empty() ? (size_t)1 : imap.size()));
+  off_tails = off_amax + al(sizeof(unsigned) * (imap.empty() ? (size_t)1 : imap.size()));
+  bytes = off_tails + al(sizeof(BatchTail) * (tails.empty() ? (size_t)1 : tails.size()));
 }
 
 void BatchTables::pack(char* hp) const {
@@ -196,6 +248,7 @@ void BatchTables::pack(char* hp) const {
   if (!imgs.empty()) std::memcpy(hp + off_imgs, imgs.data(), sizeof(BatchImage) * imgs.size());
   if (!imap.empty()) std::memcpy(hp + off_imap, imap.data(), sizeof(int) * imap.size());
   std::memcpy(hp + off_gain, gain.data(), sizeof(float) * gain.size());
+  if (!tails.empty()) std::memcpy(hp + off_tails, tails.data(), sizeof(BatchTail) * tails.size());
 }
 
 void BatchTables::launch(char* dev, hipStream_t st) const {
@@ -206,16 +259,25 @@ void BatchTables::launch(char* dev, hipStream_t st) const {
     LAUNCH(launch_batch_maxabs(im, reinterpret_cast<const int*>(dev + off_imap), (int)imap.size(), amax, st);
            launch_batch_scales(im, n, amax, reinterpret_cast<const float*>(dev + off_gain), st));
   }
+  auto tail = [&] {
+    if (tails.empty()) return;
+    ProfScope ps(fwd ? 12 : 13, -1, st);
+    LAUNCH(launch_batch_tail(fwd, storage, (int)tails.size(), tail_lds,
+                             reinterpret_cast<const BatchTail*>(dev + off_tails), cf, st));
+  };
+  if (!fwd) tail();  // the inverse starts at the coarsest levels
   for (const BatchLevel& lv : levels) {
     ProfScope ps(fwd ? 8 : 9, lv.level, st);
     LAUNCH(launch_batch_level(fwd, storage, lv.S, lv.nblocks,
                               reinterpret_cast<const BatchItem*>(dev + off_items) + lv.items_off,
                               reinterpret_cast<const int*>(dev + off_bmap) + lv.blocks_off, cf, st));
   }
+  if (fwd) tail();
 }
 
 struct BatchPlan {
   std::vector<dwt_gpu_image> desc;  // as given, levels clamped
+  int config_gen = 0;               // g_batch_config_gen when built
   BatchTables t;
   void* dev = nullptr;
   void* pinned = nullptr;
@@ -242,6 +304,7 @@ static BatchPlan* batch_plan_create(int wavelet, int direction, int storage,
     total += image_scratch_bytes(images[i]);
   }
   pl->desc.assign(images, images + n);
+  pl->config_gen = g_batch_config_gen.load();
   if (hipMalloc(&pl->scratch, std::max<size_t>(total, 256)) != hipSuccess) {
     (void)hipGetLastError();
     pl->scratch = nullptr;
@@ -287,7 +350,7 @@ static PlanRef batch_plan_cached(int wavelet, int direction, int storage, dwt_gp
   for (size_t i = 0; i < g_batch_cache->size(); ++i) {
     PlanRef p = (*g_batch_cache)[i];
     if (p->t.wavelet != wavelet || p->t.fwd != (direction == DWT_GPU_FORWARD) ||
-        p->t.storage != storage || p->t.n != n)
+        p->t.storage != storage || p->t.n != n || p->config_gen != g_batch_config_gen.load())
       continue;
     bool same = true;
     for (int k = 0; k < n && same; ++k) {
@@ -360,12 +423,23 @@ int dwt_gpu_batch_2d(int wavelet, int direction, int storage, dwt_gpu_image* ima
   }
 }
 
+int dwt_gpu_batch_tail_set(long long max_region_bytes) {
+  if (max_region_bytes < 0 || max_region_bytes > kBatchTailMax) {
+    g_err = "dwt_gpu_batch_tail_set: 0 (off) .. 131072 bytes";
+    return DWT_GPU_EINVAL;
+  }
+  g_batch_tail_bytes = max_region_bytes;
+  ++g_batch_config_gen;
+  return DWT_GPU_OK;
+}
+
 int dwt_gpu_batch_rows_set(int level, int seg_rows) {
   if (level < 0 || level >= 32 || !(seg_rows == 0 || seg_rows == 8 || seg_rows == 16 || seg_rows == 32)) {
     g_err = "dwt_gpu_batch_rows_set: level 0..31, rows 0, 8, 16 or 32";
     return DWT_GPU_EINVAL;
   }
   g_batch_rows[level] = seg_rows;
+  ++g_batch_config_gen;
   return DWT_GPU_OK;
 }
 

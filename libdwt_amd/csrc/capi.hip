@@ -343,6 +343,7 @@ void init_once() {
     g_init_status = configure_line_kernels();
     if (g_init_status == hipSuccess) g_init_status = configure_tail_kernels();
     if (g_init_status == hipSuccess) g_init_status = configure_mid_kernels();
+    if (g_init_status == hipSuccess) g_init_status = configure_batch_tail();
   });
   if (g_init_status != hipSuccess)
     fail(DWT_GPU_EHIP, "raising the kernels' LDS limits: %s", hipGetErrorString(g_init_status));

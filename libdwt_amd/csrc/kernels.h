@@ -134,6 +134,8 @@ void launch_mid_inv(const T* llsrc, long long lp, const T* det, long long dp, T*
                     int W0, int H0, int nl, const Coeffs<T>& cf, hipStream_t st);
 size_t mid_lds_bytes(bool fwd, int nl, int nsteps, int elem);
 hipError_t configure_mid_kernels();
+// Raise the batched tail kernels' dynamic LDS limit (batch.hip).
+hipError_t configure_batch_tail();
 // Device-side compare of two images (verify.hip): scratch of
 // compare_scratch_bytes(W * H) bytes; the result, compare_result_bytes() at
 // compare_result_offset(W * H), is decoded by compare_result after a copy to

@@ -182,3 +182,36 @@ def test_batch_rejects_bad_descriptors():
     one = torch.zeros((1, 8), device="cuda")
     arr = libdwt_amd.batch_descriptors([one], [u[:1]], -1)
     assert L.dwt_gpu_batch_2d(0, 0, 0, arr, 1, None) == -1  # height 1
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_batch_tail_threshold_changes_nothing(storage):
+    """The batched tail (every image's small levels in one launch, in LDS) off,
+    at its default and at its largest region give bit-identical coefficients,
+    scales and reconstructions: its arithmetic and 16-bit conversion are the
+    per-level launches'."""
+    L = libdwt_amd.lib()
+    imgs = images(21) + [np.random.default_rng(4).standard_normal((181, 181)).astype(np.float32)]
+    ts = [torch.from_numpy(x).cuda() for x in imgs]
+    results = []
+    try:
+        for tail in (0, 65536, 131072):
+            assert L.dwt_gpu_batch_tail_set(tail) == 0
+            say(f"batch {storage} tail {tail}")
+            c, s, done = libdwt_amd.batch_forward(ts, "cdf97", storage, -1)
+            r = libdwt_amd.batch_inverse(c, done, "cdf97", storage, s)
+            torch.cuda.synchronize()
+            results.append(([x.cpu() for x in c], None if s is None else [x.cpu() for x in s],
+                            [x.cpu() for x in r]))
+    finally:
+        assert L.dwt_gpu_batch_tail_set(65536) == 0
+    base = results[0]
+    for other in results[1:]:
+        for a, b in zip(base[0], other[0]):
+            assert torch.equal(a, b)
+        if base[1] is not None:
+            for a, b in zip(base[1], other[1]):
+                assert torch.equal(a, b)
+        for a, b in zip(base[2], other[2]):
+            assert torch.equal(a, b)
+    assert L.dwt_gpu_batch_tail_set(131073) == -1
