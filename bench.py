@@ -143,6 +143,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batch: tune segment rows per level and the tail threshold for the batch first")
     ap.add_argument("--no-batched", action="store_true",
                     help="image workload: skip the batched / serving / drop-in 1D legs")
     ap.add_argument("--schedule", type=int, default=0, choices=(0, 1),
@@ -157,9 +159,9 @@ def parse():
                          "images of uneven sizes, batched launches vs one launch per image; "
                          "serve: Poisson request stream through the serving loop; lines: "
                          "16384 random walks of 8192 samples, full-depth 1D")
-    ap.add_argument("--rate", type=float, default=30000.0,
+    ap.add_argument("--rate", type=float, default=120000.0,
                     help="serve: offered request rate (requests/s)")
-    ap.add_argument("--requests", type=int, default=3000, help="serve: requests per run")
+    ap.add_argument("--requests", type=int, default=6000, help="serve: requests per run")
     ap.add_argument("--client", default="native", choices=("native", "python"),
                     help="serve: submit from the library's native replay loop or from Python")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -228,7 +230,7 @@ def _timed(torch, stream, fn, steps):
 
 
 def measure_batch(wv, steps, warm, torch, libdwt_amd, workloads, storages=("f32", "f16", "bf16"), graph=True,
-                  nerr=32):
+                  nerr=32, autotune=False):
     """256 float32 images, each side log-uniform in [64, 2048] (seed 4),
     J = min(8, floor(log2(min side))), forward then inverse: batched launches
     (FP32 / FP16 / BF16 coefficient storage, eagerly and as a replayed HIP
@@ -247,6 +249,10 @@ def measure_batch(wv, steps, warm, torch, libdwt_amd, workloads, storages=("f32"
     sp = stream.cuda_stream
     L = libdwt_amd.lib()
     steps, warm = max(1, steps), max(1, warm)
+    tuned = None
+    if autotune:  # segment rows per level and the tail threshold for this batch
+        from libdwt_amd import autotune as at
+        tuned = at.tune_batch(imgs, wv, "f32", reps=3, levels=levels)
 
     # one drop-in style transform per image (in place on working copies)
     work = [t.clone() for t in imgs]
@@ -361,14 +367,17 @@ def measure_batch(wv, steps, warm, torch, libdwt_amd, workloads, storages=("f32"
     alg = sum(2 * workloads.algorithmic_bytes(w, h, j) for (w, h), j in zip(sizes, levels))
     f32 = modes["f32"]
     best_ms = min(f32["ms_per_step"], f32.get("ms_per_step_graph", f32["ms_per_step"]))
+    if tuned is not None:
+        from libdwt_amd import autotune as at
+        at.reset_batch()
     return {"sizes": sizes, "area": area, "alg_bytes": alg, "plain_ms": plain_ms, "best_ms": best_ms,
-            "modes": modes, "steps": steps, "warm": warm}
+            "modes": modes, "steps": steps, "warm": warm, "autotune": tuned}
 
 
 def bench_batch(args, torch, libdwt_amd, workloads):
     """The batch workload on its own (see measure_batch)."""
     wv = args.wavelet
-    r = measure_batch(wv, args.steps, args.warmup, torch, libdwt_amd, workloads)
+    r = measure_batch(wv, args.steps, args.warmup, torch, libdwt_amd, workloads, autotune=args.autotune)
     sizes, area, alg, plain_ms, best_ms, modes = (r["sizes"], r["area"], r["alg_bytes"], r["plain_ms"],
                                                    r["best_ms"], r["modes"])
     steps, warm = r["steps"], r["warm"]
@@ -388,6 +397,8 @@ def bench_batch(args, torch, libdwt_amd, workloads):
         "hbm_frac_f32": round(alg / (best_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "modes": modes,
     }
+    if r["autotune"] is not None:
+        out["autotune"] = {k: r["autotune"][k] for k in ("rows", "tail_bytes", "timing_us")}
     print(json.dumps(out), flush=True)
 
 

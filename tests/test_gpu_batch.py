@@ -215,3 +215,26 @@ def test_batch_tail_threshold_changes_nothing(storage):
         for a, b in zip(base[2], other[2]):
             assert torch.equal(a, b)
     assert L.dwt_gpu_batch_tail_set(131073) == -1
+
+
+def test_batch_autotune_installs_a_valid_table():
+    """The batch autotuner times rows per level and tail thresholds on a small
+    batch, installs its choice, and the batch still matches the oracle bit
+    for bit under it."""
+    from libdwt_amd import autotune
+    from oracle.oracle import oracle
+    imgs = images(17)
+    ts = [torch.from_numpy(x).cuda() for x in imgs]
+    say("batch autotune: 10 images")
+    try:
+        table = autotune.tune_batch(ts, "cdf97", "f32", reps=1)
+        assert table["tail_bytes"] in autotune.BATCH_TAILS
+        assert all(r in autotune.BATCH_ROWS for r in table["rows"].values())
+        coefs, _, done = libdwt_amd.batch_forward(ts, "cdf97", "f32", -1)
+        torch.cuda.synchronize()
+        o = oracle()
+        for x, c, j in zip(imgs, coefs, done):
+            ref, _ = o.fwd2(x, "cdf97", j)
+            np.testing.assert_array_equal(c.cpu().numpy(), ref, err_msg=f"{x.shape} J={j}")
+    finally:
+        autotune.reset_batch()
