@@ -140,8 +140,9 @@ __device__ __forceinline__ void lift_inv(T (&e)[PPL], T (&o)[PPL], int j0, int n
 // One pass of a level over `nlines` lines of length n: line i starts at
 // base + i * lstep, its samples `es` apart.  Forward: the line's samples are
 // lifted and its L coefficients written to positions [0, ceil(n/2)), its H
-// coefficients after them.  Inverse: L from [0, ceil(n/2)) and H after them
-// are lifted back to samples in order.  A line of at most 64 pairs takes LPL
+// coefficients from position hoff on (ceil(n/2) in a packed frame; the outer
+// frame's half in a sparse one, src/libdwt.c:6354-6362).  Inverse: L from
+// [0, ceil(n/2)) and H from hoff are lifted back to samples [0, n).  A line of at most 64 pairs takes LPL
 // lanes (the power of two >= its pair count), so a wavefront lifts 64 / LPL
 // lines at once; the DPP neighbour across two such lines is never used, as a
 // line's end lanes take their mirror values.  A line is read completely
@@ -173,8 +174,8 @@ struct LinePass {
 };
 
 template <bool FWD, typename T, int NS, int PPL>
-__device__ __forceinline__ void block_pass(Lds<T>* base, int lstep, int es, int nlines, int n, int lshift,
-                                           const Coeffs<T>& cf) {
+__device__ __forceinline__ void block_pass(Lds<T>* base, int lstep, int es, int nlines, int n, int hoff,
+                                           int lshift, const Coeffs<T>& cf) {
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   LinePass<FWD, T, NS, PPL> lp;
@@ -190,9 +191,9 @@ __device__ __forceinline__ void block_pass(Lds<T>* base, int lstep, int es, int 
   for (int q = 0; q < PPL; ++q) {
     const int je = min(lp.j0 + q, lp.nl - 1), jo = min(lp.j0 + q, lp.nh - 1);
     lp.ld_e[q] = (FWD ? 2 * je : je) * es;
-    lp.ld_o[q] = (FWD ? 2 * jo + 1 : lp.nl + jo) * es;
+    lp.ld_o[q] = (FWD ? 2 * jo + 1 : hoff + jo) * es;
     lp.st_e[q] = (FWD ? je : 2 * je) * es;
-    lp.st_o[q] = (FWD ? lp.nl + jo : 2 * jo + 1) * es;
+    lp.st_o[q] = (FWD ? hoff + jo : 2 * jo + 1) * es;
   }
   const int sub_off = sub * lstep;
   // two lines per iteration (independent chains, interleaved by the compiler);
@@ -215,19 +216,41 @@ __device__ __forceinline__ int block_ppl(int n) {
   return pairs <= kWave ? 1 : pairs <= 2 * kWave ? 2 : 4;
 }
 
+// hoff < 0: the packed frame's ceil(n/2).
 template <bool FWD, typename T, int NS>
 __device__ __forceinline__ void block_pass_any(Lds<T>* base, int lstep, int es, int nlines, int n,
-                                               const Coeffs<T>& cf) {
+                                               const Coeffs<T>& cf, int hoff = -1) {
+  if (hoff < 0) hoff = (n + 1) >> 1;
   const int ppl = block_ppl(n);
   if (ppl == 1) {
     const int pairs = (n + 1) >> 1;
     int lshift = 0;
     while ((1 << lshift) < pairs) ++lshift;
-    block_pass<FWD, T, NS, 1>(base, lstep, es, nlines, n, lshift, cf);
+    block_pass<FWD, T, NS, 1>(base, lstep, es, nlines, n, hoff, lshift, cf);
   } else if (ppl == 2) {
-    block_pass<FWD, T, NS, 2>(base, lstep, es, nlines, n, 6, cf);
+    block_pass<FWD, T, NS, 2>(base, lstep, es, nlines, n, hoff, 6, cf);
   } else {
-    block_pass<FWD, T, NS, 4>(base, lstep, es, nlines, n, 6, cf);
+    block_pass<FWD, T, NS, 4>(base, lstep, es, nlines, n, hoff, 6, cf);
+  }
+}
+
+// Zero padding of one level of a sparse frame held in LDS (C, pitch pc), outer
+// W x H, inner Wi x Hi (src/libdwt.c:6377-6397 forward, 6729-6745 inverse).
+// Forward: every row's samples between its L part and the outer half and past
+// its H part, then every column's likewise.  Inverse: everything past the
+// inner size.  Followed by a barrier in the caller.
+template <bool FWD, typename T>
+__device__ __forceinline__ void block_zero_pad(Lds<T>* C, int pc, int W, int H, int Wi, int Hi) {
+  const int Wd = (W + 1) >> 1, Hd = (H + 1) >> 1;
+  for (int i = threadIdx.x; i < W * H; i += blockDim.x) {
+    const int y = i / W, x = i - y * W;
+    bool z;
+    if (FWD)
+      z = (x >= ((Wi + 1) >> 1) && x < Wd) || x >= Wd + (Wi >> 1) ||
+          (y >= ((Hi + 1) >> 1) && y < Hd) || y >= Hd + (Hi >> 1);
+    else
+      z = x >= Wi || y >= Hi;
+    if (z) C[y * pc + x] = T(0);
   }
 }
 

@@ -53,6 +53,61 @@ block_inv_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, l
   block_copy<false>(C, pc, dst, dp, W0, H0);
 }
 
+// Sparse frames (outer W0 x H0, inner Wi0 x Hi0 at the region's first level;
+// src/libdwt.c:6331-6400, 6683-6747): every level lifts the outer frame's rows
+// over the inner length with H at the outer half, then its columns likewise,
+// garbage rows and columns included, exactly as the reference's line loops
+// do; optional zero padding after each level.  In place on one region.
+template <typename T, int NS>
+__global__ void __launch_bounds__(kBlockThreads)
+block_sparse_fwd_kernel(T* __restrict__ a, long long ap, int W0, int H0, int Wi0, int Hi0, int nlev, int zp,
+                        Coeffs<T> cf) {
+  extern __shared__ __align__(16) unsigned char blk_smem[];
+  Lds<T>* C = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
+  const int pc = block_pitch(W0);
+  block_copy<true>(C, pc, a, ap, W0, H0);
+  __syncthreads();
+  for (int lev = 0; lev < nlev; ++lev) {
+    const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
+    const int Wi = ceil_div_pow2(Wi0, lev), Hi = ceil_div_pow2(Hi0, lev);
+    block_pass_any<true, T, NS>(C, pc, 1, H, Wi, cf, (W + 1) >> 1);  // rows y < H, length Wi
+    __syncthreads();
+    block_pass_any<true, T, NS>(C, 1, pc, W, Hi, cf, (H + 1) >> 1);  // columns x < W, length Hi
+    __syncthreads();
+    if (zp) {
+      block_zero_pad<true, T>(C, pc, W, H, Wi, Hi);
+      __syncthreads();
+    }
+  }
+  block_copy<false>(C, pc, a, ap, W0, H0);
+}
+
+template <typename T, int NS>
+__global__ void __launch_bounds__(kBlockThreads)
+block_sparse_inv_kernel(T* __restrict__ a, long long ap, int W0, int H0, int Wi0, int Hi0, int nlev, int zp,
+                        Coeffs<T> cf) {
+  extern __shared__ __align__(16) unsigned char blk_smem[];
+  Lds<T>* C = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
+  const int pc = block_pitch(W0);
+  block_copy<true>(C, pc, a, ap, W0, H0);
+  __syncthreads();
+  for (int lev = nlev - 1; lev >= 0; --lev) {
+    // output level lev: outer W x H, inner Wi x Hi; H parts at the coarser
+    // level's outer size
+    const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
+    const int Wi = ceil_div_pow2(Wi0, lev), Hi = ceil_div_pow2(Hi0, lev);
+    block_pass_any<false, T, NS>(C, pc, 1, H, Wi, cf, (W + 1) >> 1);  // rows y < H
+    __syncthreads();
+    block_pass_any<false, T, NS>(C, 1, pc, W, Hi, cf, (H + 1) >> 1);  // columns x < W
+    __syncthreads();
+    if (zp) {
+      block_zero_pad<false, T>(C, pc, W, H, Wi, Hi);
+      __syncthreads();
+    }
+  }
+  block_copy<false>(C, pc, a, ap, W0, H0);
+}
+
 // ---- host side ----------------------------------------------------------------
 // The block kernel takes over from the first level j >= 1 whose W x H region
 // is at most g_block_bytes, or from level 0 when the whole image is at most
@@ -86,7 +141,15 @@ int tail_start(int W, int H, int L, int elem) {
 }
 
 hipError_t configure_tail_kernels() {
-  const void* fns[] = {reinterpret_cast<const void*>(&block_fwd_kernel<float, 4>),
+  const void* fns[] = {reinterpret_cast<const void*>(&block_sparse_fwd_kernel<float, 4>),
+                       reinterpret_cast<const void*>(&block_sparse_fwd_kernel<float, 2>),
+                       reinterpret_cast<const void*>(&block_sparse_fwd_kernel<double, 4>),
+                       reinterpret_cast<const void*>(&block_sparse_fwd_kernel<double, 2>),
+                       reinterpret_cast<const void*>(&block_sparse_inv_kernel<float, 4>),
+                       reinterpret_cast<const void*>(&block_sparse_inv_kernel<float, 2>),
+                       reinterpret_cast<const void*>(&block_sparse_inv_kernel<double, 4>),
+                       reinterpret_cast<const void*>(&block_sparse_inv_kernel<double, 2>),
+                       reinterpret_cast<const void*>(&block_fwd_kernel<float, 4>),
                        reinterpret_cast<const void*>(&block_fwd_kernel<float, 2>),
                        reinterpret_cast<const void*>(&block_fwd_kernel<double, 4>),
                        reinterpret_cast<const void*>(&block_fwd_kernel<double, 2>),
@@ -138,6 +201,48 @@ void launch_block_inv(const T* src, long long sp, T* dst, long long dp, int W0, 
     hipLaunchKernelGGL((block_inv_kernel<T, 2>), grid, dim3(kBlockThreads), lds, st, src, sp, dst, dp, W0,
                        H0, nlev, cf, rc);
 }
+
+// Sparse tail: the first level j >= 1 (or 0 for a whole small frame) whose
+// outer region fits the block kernel (the packed thresholds), and from which
+// every remaining level has inner sides of at least 2 (lines of one sample
+// take the line kernel's N == 1 path); L when there is none.
+int sparse_tail_start(int Wo, int Ho, int Wi, int Hi, int L, int elem) {
+  if (g_block_bytes <= 0) return L;
+  int jt = L;
+  for (int j = L - 1; j >= 0; --j) {
+    if (ceil_div_pow2(Wi, j) < 2 || ceil_div_pow2(Hi, j) < 2) break;
+    const long long wj = ceil_div_pow2(Wo, j), hj = ceil_div_pow2(Ho, j);
+    const long long limit = j == 0 ? 2 * g_block_bytes : g_block_bytes;
+    if (wj * hj * elem <= limit && wj <= kBlockMaxLine && hj <= kBlockMaxLine &&
+        block_lds_bytes(wj, hj, elem) <= kBlockLdsMax)
+      jt = j;
+    else
+      break;
+  }
+  return jt;
+}
+
+template <typename T>
+void launch_block_sparse(bool fwd, T* a, long long ap, int W0, int H0, int Wi0, int Hi0, int nlev, int zp,
+                         const Coeffs<T>& cf, hipStream_t st) {
+  const size_t lds = (size_t)block_lds_bytes(W0, H0, (int)sizeof(T));
+  const dim3 grid(1), block(kBlockThreads);
+  if (fwd) {
+    if (cf.nsteps == 4)
+      hipLaunchKernelGGL((block_sparse_fwd_kernel<T, 4>), grid, block, lds, st, a, ap, W0, H0, Wi0, Hi0, nlev, zp, cf);
+    else
+      hipLaunchKernelGGL((block_sparse_fwd_kernel<T, 2>), grid, block, lds, st, a, ap, W0, H0, Wi0, Hi0, nlev, zp, cf);
+  } else {
+    if (cf.nsteps == 4)
+      hipLaunchKernelGGL((block_sparse_inv_kernel<T, 4>), grid, block, lds, st, a, ap, W0, H0, Wi0, Hi0, nlev, zp, cf);
+    else
+      hipLaunchKernelGGL((block_sparse_inv_kernel<T, 2>), grid, block, lds, st, a, ap, W0, H0, Wi0, Hi0, nlev, zp, cf);
+  }
+}
+template void launch_block_sparse<float>(bool, float*, long long, int, int, int, int, int, int, const Coeffs<float>&,
+                                         hipStream_t);
+template void launch_block_sparse<double>(bool, double*, long long, int, int, int, int, int, int,
+                                          const Coeffs<double>&, hipStream_t);
 
 template void launch_block_fwd<float>(const float*, long long, float*, long long, int, int, int,
                                       const Coeffs<float>&, hipStream_t, const RiderCopy<float>*);

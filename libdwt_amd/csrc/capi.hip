@@ -1184,6 +1184,15 @@ static bool packed_layout(const char* ptr, long long sx, long long sy, int Wo, i
          (reinterpret_cast<uintptr_t>(ptr) % sizeof(T)) == 0 && Wo == Wi && Ho == Hi;
 }
 
+// A sparse frame (outer != inner) with contiguous rows at an element pitch:
+// its small levels can run in LDS (launch_block_sparse).
+template <typename T>
+static bool sparse_rows_layout(const char* ptr, long long sx, long long sy, int Wo, int Ho, int Wi, int Hi) {
+  return sy == (long long)sizeof(T) && sx > 0 && sx % (long long)sizeof(T) == 0 &&
+         sx >= (long long)Wo * (long long)sizeof(T) && (reinterpret_cast<uintptr_t>(ptr) % sizeof(T)) == 0 &&
+         (Wo != Wi || Ho != Hi);
+}
+
 template <typename T>
 static int fast_levels(int W, int H, int J) {
   int jf = 0;
@@ -1255,7 +1264,16 @@ static void fwd2_device(char* ptr, long long sx, long long sy, int Wo, int Ho, i
   int jf = 0;
   if (packed_layout<T>(ptr, sx, sy, Wo, Ho, Wi, Hi)) jf = fast_levels<T>(Wo, Ho, J);
   if (jf > 0) fast_fwd<T>(reinterpret_cast<T*>(ptr), sx / (long long)sizeof(T), Wo, Ho, jf, cf, sc, st);
-  for (int j = jf; j < J; ++j) generic_fwd_level<T>(ptr, sx, sy, Wo, Ho, Wi, Hi, j, zp, cf, sc, st);
+  // sparse frames: the small levels in one block launch
+  const int jt = jf == 0 && sparse_rows_layout<T>(ptr, sx, sy, Wo, Ho, Wi, Hi)
+                     ? sparse_tail_start(Wo, Ho, Wi, Hi, J, (int)sizeof(T)) : J;
+  for (int j = jf; j < jt; ++j) generic_fwd_level<T>(ptr, sx, sy, Wo, Ho, Wi, Hi, j, zp, cf, sc, st);
+  if (jt < J) {
+    ProfScope ps(4, jt, st);
+    LAUNCH(launch_block_sparse<T>(true, reinterpret_cast<T*>(ptr), sx / (long long)sizeof(T), ceil_div_pow2(Wo, jt),
+                                  ceil_div_pow2(Ho, jt), ceil_div_pow2(Wi, jt), ceil_div_pow2(Hi, jt), J - jt, zp, cf,
+                                  st));
+  }
 }
 
 template <typename T>
@@ -1277,7 +1295,16 @@ static void inv2_device(char* ptr, long long sx, long long sy, int Wo, int Ho, i
   const Coeffs<T> cf = make_coeffs<T>(w, false);
   int jf = 0;
   if (packed_layout<T>(ptr, sx, sy, Wo, Ho, Wi, Hi)) jf = fast_levels<T>(Wo, Ho, J);
-  for (int j = J; j > jf; --j) generic_inv_level<T>(ptr, sx, sy, Wo, Ho, Wi, Hi, j, zp, cf, sc, st);
+  // sparse frames: the coarsest levels in one block launch
+  const int jt = jf == 0 && sparse_rows_layout<T>(ptr, sx, sy, Wo, Ho, Wi, Hi)
+                     ? sparse_tail_start(Wo, Ho, Wi, Hi, J, (int)sizeof(T)) : J;
+  if (jt < J) {
+    ProfScope ps(5, jt, st);
+    LAUNCH(launch_block_sparse<T>(false, reinterpret_cast<T*>(ptr), sx / (long long)sizeof(T), ceil_div_pow2(Wo, jt),
+                                  ceil_div_pow2(Ho, jt), ceil_div_pow2(Wi, jt), ceil_div_pow2(Hi, jt), J - jt, zp, cf,
+                                  st));
+  }
+  for (int j = jt; j > jf; --j) generic_inv_level<T>(ptr, sx, sy, Wo, Ho, Wi, Hi, j, zp, cf, sc, st);
   if (jf > 0) fast_inv<T>(reinterpret_cast<T*>(ptr), sx / (long long)sizeof(T), Wo, Ho, jf, cf, sc, st);
 }
 

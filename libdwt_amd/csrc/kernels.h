@@ -118,6 +118,13 @@ template <typename T>
 void launch_block_inv(const T* src, long long sp, T* dst, long long dp, int W0, int H0, int nlev,
                       const Coeffs<T>& cf, hipStream_t st, const RiderCopy<T>* rider = nullptr);
 int tail_start(int W, int H, int L, int elem);
+// Sparse frames (outer Wo x Ho, inner Wi x Hi, contiguous rows): levels
+// [sparse_tail_start, L) as one block launch, in place on the region at the
+// frame's origin (outer W0 x H0 and inner Wi0 x Hi0 at that level).
+int sparse_tail_start(int Wo, int Ho, int Wi, int Hi, int L, int elem);
+template <typename T>
+void launch_block_sparse(bool fwd, T* a, long long ap, int W0, int H0, int Wi0, int Hi0, int nlev, int zp,
+                         const Coeffs<T>& cf, hipStream_t st);
 void tail_config(long long max_region_bytes, int unused);
 hipError_t configure_tail_kernels();
 // Fused middle levels (mid_kernels.hip): nl (<= kMidMaxLevels) consecutive
