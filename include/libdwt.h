@@ -423,6 +423,12 @@ int dwt_gpu_mid_set(long long max_level_bytes);
  * as soon as the rows they read are finished (dataflow, no barrier between
  * levels). */
 int dwt_gpu_schedule_set(int mode);
+/* Sparse frames (size_o != size_i) whose rows are contiguous: 1 (default)
+ * runs each large level as one launch (inner tiles, garbage rows and columns
+ * as the reference transforms them, into scratch; copied back by riders) and
+ * the small levels as one block launch; 0 runs the large levels as the
+ * generic per-line passes.  The results are identical. */
+int dwt_gpu_sparse_set(int fused);
 /* Synchronise `stream` and report whether a persistent launch of this thread
  * on it hit its wait time limit since the last check (DWT_GPU_EHIP; the
  * results of that launch are then invalid).  The drop-in calls check this

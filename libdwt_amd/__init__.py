@@ -149,6 +149,8 @@ def lib() -> C.CDLL:
         L.dwt_gpu_batch_rows_set.restype = i
         L.dwt_gpu_batch_tail_set.argtypes = [C.c_longlong]
         L.dwt_gpu_batch_tail_set.restype = i
+        L.dwt_gpu_sparse_set.argtypes = [i]
+        L.dwt_gpu_sparse_set.restype = i
         L.dwt_gpu_1d_batch.argtypes = [i, i, i, vp, C.c_longlong, C.c_longlong, i, i,
                                        C.POINTER(i), vp]
         L.dwt_gpu_1d_batch.restype = i

@@ -19,7 +19,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libdwt_gpu.so")
 OBJDIR = os.path.join(HERE, "_build")
 SOURCES = ["line_kernels.hip", "tile_kernels.hip", "block_kernels.hip", "inplace.hip", "mid_kernels.hip", "pyr_kernels.hip", "batch.hip", "capi.hip",
-           "batch_host.hip", "server.hip", "lines.hip", "util.hip", "verify.hip"]
+           "batch_host.hip", "server.hip", "lines.hip", "util.hip", "verify.hip", "sparse.hip"]
 ARCH = os.environ.get("DWT_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

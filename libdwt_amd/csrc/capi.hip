@@ -360,6 +360,9 @@ static bool aligned_pair_host(const void* p, long long pitch) {
 // Schedule of the levels after level 0: 0 = one launch per level plus the
 // block kernel for the small ones (default), 1 = one pyramid launch.
 static int g_schedule = 0;
+// Sparse frames with contiguous rows: 1 = the large levels as fused sparse
+// level launches (default), 0 = the generic per-line passes.
+static int g_sparse_fused = 1;
 static int g_trace_items = 0;  // > 0: pyramid launches record a per-item timeline
 static int g_pyr_grid_cap = 0;  // > 0: at most this many workgroups per pyramid launch
 static constexpr size_t kMaxPlans = 8;
@@ -1193,6 +1196,105 @@ static bool sparse_rows_layout(const char* ptr, long long sx, long long sy, int 
          (Wo != Wi || Ho != Hi);
 }
 
+// Levels [0, n) of a sparse frame whose inner sides stay >= 2: the fused
+// sparse level launches take them.
+static int sparse_fused_levels(int Wi, int Hi, int J) {
+  int j = 0;
+  while (j < J && ceil_div_pow2(Wi, j) >= 2 && ceil_div_pow2(Hi, j) >= 2) ++j;
+  return j;
+}
+
+// Copy of a level's W x H output region from scratch into the frame, except
+// the next level's Wn x Hn region (0 x 0: all of it), as a RiderCopy.
+template <typename T>
+static RiderCopy<T> region_copy(const T* src, long long sp, T* dst, long long dp, int W, int H, int Wn, int Hn) {
+  RiderCopy<T> rc{};
+  rc.src = src;
+  rc.sp = sp;
+  rc.dst = dst;
+  rc.dp = dp;
+  rc.xa = Hn > 0 ? Wn : 0;
+  rc.wa = W - rc.xa;
+  rc.ha = Hn > 0 ? Hn : H;
+  rc.wb = W;
+  rc.hb = H - rc.ha;
+  rc.r0 = 0;
+  rc.r1 = H;
+  rc.blocks = std::min(H, 2048);
+  constexpr int per = 16 / (int)sizeof(T);
+  rc.vec = (reinterpret_cast<uintptr_t>(src) % 16) == 0 && (reinterpret_cast<uintptr_t>(dst) % 16) == 0 &&
+           sp % per == 0 && dp % per == 0 && rc.xa % per == 0;
+  return rc;
+}
+
+// Forward levels [0, jn) of a sparse frame with contiguous rows (A, element
+// pitch p), each one launch reading the previous level's output and writing
+// its own to scratch (Q0: level 0's outer size, Q1: level 1's, alternating).
+// The part of a level's output the next level does not rewrite is copied into
+// the frame by riders on the next launch; the last level's whole region by a
+// copy of its own, so that what follows (the block tail or the generic levels)
+// finds the frame complete.
+template <typename T>
+static void sparse_fwd_levels(T* A, long long p, int Wo, int Ho, int Wi, int Hi, int jn, int zp, const Coeffs<T>& cf,
+                              Scratch& sc, hipStream_t st) {
+  T* Q[2];
+  long long Qp[2];
+  for (int i = 0; i < 2 && i < jn; ++i) {
+    Qp[i] = (long long)pad_pitch(ceil_div_pow2(Wo, i));
+    Q[i] = (T*)sc.get(i == 0 ? 2 : 1, sizeof(T) * (size_t)Qp[i] * ceil_div_pow2(Ho, i));
+  }
+  const T* in = A;
+  long long ip = p;
+  RiderCopy<T> pending{};
+  for (int j = 0; j < jn; ++j) {
+    const int W = ceil_div_pow2(Wo, j), H = ceil_div_pow2(Ho, j);
+    T* q = Q[j & 1];
+    {
+      ProfScope ps(0, j, st);
+      LAUNCH(launch_sparse_fwd<T>(in, ip, q, Qp[j & 1], W, H, ceil_div_pow2(Wi, j), ceil_div_pow2(Hi, j), zp, cf,
+                                  &pending, st));
+    }
+    pending = region_copy<T>(q, Qp[j & 1], A, p, W, H, (W + 1) / 2, (H + 1) / 2);
+    in = q;
+    ip = Qp[j & 1];
+  }
+  if (jn > 0) {
+    const int W = ceil_div_pow2(Wo, jn - 1), H = ceil_div_pow2(Ho, jn - 1);
+    const RiderCopy<T> all = region_copy<T>(Q[(jn - 1) & 1], Qp[(jn - 1) & 1], A, p, W, H, 0, 0);
+    LAUNCH(launch_rider_alone<T>(all, st));
+  }
+}
+
+// Inverse output levels jn - 1 .. 0 of a sparse frame (the coarser levels are
+// in the frame already), each one launch reading the previous level's output
+// and the frame's details and writing scratch; the last output is copied into
+// the frame.
+template <typename T>
+static void sparse_inv_levels(T* A, long long p, int Wo, int Ho, int Wi, int Hi, int jn, int zp, const Coeffs<T>& cf,
+                              Scratch& sc, hipStream_t st) {
+  if (jn <= 0) return;
+  T* Q[2];
+  long long Qp[2];
+  for (int i = 0; i < 2 && i < jn; ++i) {
+    Qp[i] = (long long)pad_pitch(ceil_div_pow2(Wo, i));
+    Q[i] = (T*)sc.get(i == 0 ? 2 : 1, sizeof(T) * (size_t)Qp[i] * ceil_div_pow2(Ho, i));
+  }
+  const T* prev = A;
+  long long pp = p;
+  for (int k = jn - 1; k >= 0; --k) {
+    T* q = Q[k & 1];
+    {
+      ProfScope ps(1, k, st);
+      LAUNCH(launch_sparse_inv<T>(prev, pp, A, p, q, Qp[k & 1], ceil_div_pow2(Wo, k), ceil_div_pow2(Ho, k),
+                                  ceil_div_pow2(Wi, k), ceil_div_pow2(Hi, k), zp, cf, st));
+    }
+    prev = q;
+    pp = Qp[k & 1];
+  }
+  const RiderCopy<T> all = region_copy<T>(Q[0], Qp[0], A, p, Wo, Ho, 0, 0);
+  LAUNCH(launch_rider_alone<T>(all, st));
+}
+
 template <typename T>
 static int fast_levels(int W, int H, int J) {
   int jf = 0;
@@ -1264,9 +1366,14 @@ static void fwd2_device(char* ptr, long long sx, long long sy, int Wo, int Ho, i
   int jf = 0;
   if (packed_layout<T>(ptr, sx, sy, Wo, Ho, Wi, Hi)) jf = fast_levels<T>(Wo, Ho, J);
   if (jf > 0) fast_fwd<T>(reinterpret_cast<T*>(ptr), sx / (long long)sizeof(T), Wo, Ho, jf, cf, sc, st);
-  // sparse frames: the small levels in one block launch
-  const int jt = jf == 0 && sparse_rows_layout<T>(ptr, sx, sy, Wo, Ho, Wi, Hi)
-                     ? sparse_tail_start(Wo, Ho, Wi, Hi, J, (int)sizeof(T)) : J;
+  // sparse frames: the large levels as fused level launches, the small ones
+  // in one block launch
+  const bool sparse = jf == 0 && sparse_rows_layout<T>(ptr, sx, sy, Wo, Ho, Wi, Hi);
+  const int jt = sparse ? sparse_tail_start(Wo, Ho, Wi, Hi, J, (int)sizeof(T)) : J;
+  if (sparse && g_sparse_fused) {
+    jf = std::min(jt, sparse_fused_levels(Wi, Hi, J));
+    sparse_fwd_levels<T>(reinterpret_cast<T*>(ptr), sx / (long long)sizeof(T), Wo, Ho, Wi, Hi, jf, zp, cf, sc, st);
+  }
   for (int j = jf; j < jt; ++j) generic_fwd_level<T>(ptr, sx, sy, Wo, Ho, Wi, Hi, j, zp, cf, sc, st);
   if (jt < J) {
     ProfScope ps(4, jt, st);
@@ -1295,16 +1402,20 @@ static void inv2_device(char* ptr, long long sx, long long sy, int Wo, int Ho, i
   const Coeffs<T> cf = make_coeffs<T>(w, false);
   int jf = 0;
   if (packed_layout<T>(ptr, sx, sy, Wo, Ho, Wi, Hi)) jf = fast_levels<T>(Wo, Ho, J);
-  // sparse frames: the coarsest levels in one block launch
-  const int jt = jf == 0 && sparse_rows_layout<T>(ptr, sx, sy, Wo, Ho, Wi, Hi)
-                     ? sparse_tail_start(Wo, Ho, Wi, Hi, J, (int)sizeof(T)) : J;
+  // sparse frames: the coarsest levels in one block launch, the large ones as
+  // fused level launches
+  const bool sparse = jf == 0 && sparse_rows_layout<T>(ptr, sx, sy, Wo, Ho, Wi, Hi);
+  const int jt = sparse ? sparse_tail_start(Wo, Ho, Wi, Hi, J, (int)sizeof(T)) : J;
+  const int jn = sparse && g_sparse_fused ? std::min(jt, sparse_fused_levels(Wi, Hi, J)) : 0;
   if (jt < J) {
     ProfScope ps(5, jt, st);
     LAUNCH(launch_block_sparse<T>(false, reinterpret_cast<T*>(ptr), sx / (long long)sizeof(T), ceil_div_pow2(Wo, jt),
                                   ceil_div_pow2(Ho, jt), ceil_div_pow2(Wi, jt), ceil_div_pow2(Hi, jt), J - jt, zp, cf,
                                   st));
   }
-  for (int j = jt; j > jf; --j) generic_inv_level<T>(ptr, sx, sy, Wo, Ho, Wi, Hi, j, zp, cf, sc, st);
+  for (int j = jt; j > std::max(jf, jn); --j) generic_inv_level<T>(ptr, sx, sy, Wo, Ho, Wi, Hi, j, zp, cf, sc, st);
+  if (jn > 0)
+    sparse_inv_levels<T>(reinterpret_cast<T*>(ptr), sx / (long long)sizeof(T), Wo, Ho, Wi, Hi, jn, zp, cf, sc, st);
   if (jf > 0) fast_inv<T>(reinterpret_cast<T*>(ptr), sx / (long long)sizeof(T), Wo, Ho, jf, cf, sc, st);
 }
 
@@ -2149,6 +2260,15 @@ int dwt_gpu_schedule_set(int mode) {
     return DWT_GPU_EINVAL;
   }
   g_schedule = mode;
+  return DWT_GPU_OK;
+}
+
+int dwt_gpu_sparse_set(int fused) {
+  if (fused != 0 && fused != 1) {
+    g_err = "dwt_gpu_sparse_set: 0 or 1";
+    return DWT_GPU_EINVAL;
+  }
+  g_sparse_fused = fused;
   return DWT_GPU_OK;
 }
 

@@ -608,7 +608,7 @@ struct GlobalOut {  // two adjacent output samples, global
       V2 v;
       v.x = e;
       v.y = o;
-      *gstore_ptr(reinterpret_cast<V2*>(rp)) = v;
+      *reinterpret_cast<Gbl<V2>*>(gstore_ptr(rp)) = v;
       return;
     }
 #endif

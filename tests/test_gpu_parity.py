@@ -390,3 +390,37 @@ def test_pyramid_repeated_launches_reuse_counters():
     finally:
         libdwt_amd.schedule_set(0)
     np.testing.assert_array_equal(t.cpu().numpy(), want)
+
+
+def test_sparse_fused_odd_pitch_and_generic_agree():
+    """Sparse frames at an odd row pitch and an odd element offset from the
+    allocation: the fused sparse level launches (default) and the generic
+    per-line passes give the oracle's result bit for bit, forward and
+    inverse, with and without zero padding."""
+    from oracle.oracle import oracle
+    o = oracle()
+    L = libdwt_amd.lib()
+    rng = np.random.default_rng(29)
+    for (Wo, Ho, Wi, Hi, pitch, off) in ((700, 300, 513, 257, 703, 1), (1030, 517, 1030, 260, 1031, 3)):
+        for zp in (0, 1):
+            base = rng.standard_normal(pitch * Ho + off + 8).astype(np.float32)
+            frame = base[off:off + pitch * Ho].reshape(Ho, pitch)
+            want = frame.copy()
+            jj = o.fwd2_raw(want, "cdf97", pitch * 4, 4, Wo, Ho, Wi, Hi, -1, 0, zp)
+            back = want.copy()
+            o.inv2_raw(back, "cdf97", pitch * 4, 4, Wo, Ho, Wi, Hi, jj, 0, zp)
+            for fused in (1, 0):
+                assert L.dwt_gpu_sparse_set(fused) == 0
+                say(f"sparse pitch {pitch} off {off} {Wo}x{Ho} in {Wi}x{Hi} zp={zp} fused={fused}")
+                t = torch.from_numpy(base.copy()).cuda()
+                ptr = t.data_ptr() + 4 * off
+                j = libdwt_amd.transform_2d_async("cdf97", True, ptr, pitch * 4, 4, Wo, Ho, Wi, Hi, j=-1,
+                                                  zero_padding=zp)
+                assert j == jj
+                got = t.cpu().numpy()[off:off + pitch * Ho].reshape(Ho, pitch)
+                np.testing.assert_array_equal(got, want, err_msg=f"forward fused={fused}")
+                libdwt_amd.transform_2d_async("cdf97", False, ptr, pitch * 4, 4, Wo, Ho, Wi, Hi, j=j,
+                                              zero_padding=zp)
+                got = t.cpu().numpy()[off:off + pitch * Ho].reshape(Ho, pitch)
+                np.testing.assert_array_equal(got, back, err_msg=f"inverse fused={fused}")
+    assert L.dwt_gpu_sparse_set(1) == 0
