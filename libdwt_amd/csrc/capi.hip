@@ -1267,8 +1267,11 @@ static void sparse_fwd_levels(T* A, long long p, int Wo, int Ho, int Wi, int Hi,
 
 // Inverse output levels jn - 1 .. 0 of a sparse frame (the coarser levels are
 // in the frame already), each one launch reading the previous level's output
-// and the frame's details and writing scratch; the last output is copied into
-// the frame.
+// and the frame's details and writing scratch.  Level 0 writes the frame
+// itself: what it reads of the frame (everything outside the next level's
+// region) is first preserved in scratch D by riders on the coarser levels'
+// launches, which neither read nor write that part.  With one level only its
+// output goes through scratch and is copied back.
 template <typename T>
 static void sparse_inv_levels(T* A, long long p, int Wo, int Ho, int Wi, int Hi, int jn, int zp, const Coeffs<T>& cf,
                               Scratch& sc, hipStream_t st) {
@@ -1279,20 +1282,37 @@ static void sparse_inv_levels(T* A, long long p, int Wo, int Ho, int Wi, int Hi,
     Qp[i] = (long long)pad_pitch(ceil_div_pow2(Wo, i));
     Q[i] = (T*)sc.get(i == 0 ? 2 : 1, sizeof(T) * (size_t)Qp[i] * ceil_div_pow2(Ho, i));
   }
+  const int Wod = (Wo + 1) / 2, Hod = (Ho + 1) / 2;
+  T* D = nullptr;
+  long long Dp = 0;
+  RiderCopy<T> keep{};
+  std::vector<int2> parts;
+  if (jn >= 2) {
+    Dp = (long long)pad_pitch(Wo);
+    D = (T*)sc.get(0, sizeof(T) * (size_t)Dp * Ho);
+    keep = region_copy<T>(A, p, D, Dp, Wo, Ho, Wod, Hod);
+    split_rows(Ho, std::vector<int>(jn - 1, 1), parts);  // carried by levels jn - 1 .. 1
+  }
   const T* prev = A;
   long long pp = p;
   for (int k = jn - 1; k >= 0; --k) {
-    T* q = Q[k & 1];
+    const bool direct = k == 0 && jn >= 2;
+    T* q = direct ? A : Q[k & 1];
+    const long long qp = direct ? p : Qp[k & 1];
+    const RiderCopy<T> part = k > 0 && jn >= 2 ? rider_part(keep, parts[jn - 1 - k], kLevelRiderCap) : RiderCopy<T>{};
     {
       ProfScope ps(1, k, st);
-      LAUNCH(launch_sparse_inv<T>(prev, pp, A, p, q, Qp[k & 1], ceil_div_pow2(Wo, k), ceil_div_pow2(Ho, k),
-                                  ceil_div_pow2(Wi, k), ceil_div_pow2(Hi, k), zp, cf, st));
+      LAUNCH(launch_sparse_inv<T>(prev, pp, direct ? D : A, direct ? Dp : p, q, qp, ceil_div_pow2(Wo, k),
+                                  ceil_div_pow2(Ho, k), ceil_div_pow2(Wi, k), ceil_div_pow2(Hi, k), zp, cf, &part,
+                                  st));
     }
     prev = q;
-    pp = Qp[k & 1];
+    pp = qp;
   }
-  const RiderCopy<T> all = region_copy<T>(Q[0], Qp[0], A, p, Wo, Ho, 0, 0);
-  LAUNCH(launch_rider_alone<T>(all, st));
+  if (jn == 1) {
+    const RiderCopy<T> all = region_copy<T>(Q[0], Qp[0], A, p, Wo, Ho, 0, 0);
+    LAUNCH(launch_rider_alone<T>(all, st));
+  }
 }
 
 template <typename T>

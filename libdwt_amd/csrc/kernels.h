@@ -126,13 +126,14 @@ int sparse_tail_start(int Wo, int Ho, int Wi, int Hi, int L, int elem);
 // >= 2) in one launch (sparse.hip).  Forward: in -> q (the level's whole
 // output region); rider: a copy carried by extra workgroups.  Inverse: the
 // previous level's output prev (covering [0, ceil(Ho/2)) x [0, ceil(Wo/2))) and
-// the frame's details -> q.
+// the frame's details (everything else) -> q; rider as in the forward.
 template <typename T>
 void launch_sparse_fwd(const T* in, long long ip, T* q, long long qp, int Wo, int Ho, int Wi, int Hi, int zp,
                        const Coeffs<T>& cf, const RiderCopy<T>* rider, hipStream_t st);
 template <typename T>
 void launch_sparse_inv(const T* prev, long long pp, const T* frame, long long fp, T* q, long long qp, int Wo,
-                       int Ho, int Wi, int Hi, int zp, const Coeffs<T>& cf, hipStream_t st);
+                       int Ho, int Wi, int Hi, int zp, const Coeffs<T>& cf, const RiderCopy<T>* rider,
+                       hipStream_t st);
 template <typename T>
 void launch_block_sparse(bool fwd, T* a, long long ap, int W0, int H0, int Wi0, int Hi0, int nlev, int zp,
                          const Coeffs<T>& cf, hipStream_t st);

@@ -107,8 +107,8 @@ __device__ __forceinline__ void sparse_rect(const SparseArgs<T>& a, int b) {
   }
 }
 
-// Workgroup index b of the parts after A: B ranges, C ranges, D rectangles,
-// then riders.  Returns true when the workgroup was one of them.
+// Workgroup b of the parts after A, in order: B ranges, C ranges, D
+// rectangles, then riders.
 template <bool FWD, typename T, int NS>
 __device__ __forceinline__ void sparse_lines(const SparseArgs<T>& a, int b, const RiderCopy<T>& rc,
                                              const Coeffs<T>& cf) {
@@ -283,7 +283,8 @@ void launch_sparse_fwd(const T* in, long long ip, T* q, long long qp, int Wo, in
 
 template <typename T>
 void launch_sparse_inv(const T* prev, long long pp, const T* frame, long long fp, T* q, long long qp, int Wo,
-                       int Ho, int Wi, int Hi, int zp, const Coeffs<T>& cf, hipStream_t st) {
+                       int Ho, int Wi, int Hi, int zp, const Coeffs<T>& cf, const RiderCopy<T>* rider,
+                       hipStream_t st) {
   SparseArgs<T> a{};
   a.in = frame;
   a.ip = fp;
@@ -333,6 +334,7 @@ void launch_sparse_inv(const T* prev, long long pp, const T* frame, long long fp
     add_rect(a, Hi, Ho, 0, Wo);
   }
   RiderCopy<T> rc{};
+  if (rider && rider->r1 > rider->r0 && rider->blocks > 0) rc = *rider;
   const dim3 grid(total_blocks(a, rc)), block(256);
   if (cf.nsteps == 4)
     hipLaunchKernelGGL((sparse_inv_kernel<T, 4>), grid, block, 0, st, a, cf, rc);
@@ -345,8 +347,10 @@ template void launch_sparse_fwd<float>(const float*, long long, float*, long lon
 template void launch_sparse_fwd<double>(const double*, long long, double*, long long, int, int, int, int, int,
                                         const Coeffs<double>&, const RiderCopy<double>*, hipStream_t);
 template void launch_sparse_inv<float>(const float*, long long, const float*, long long, float*, long long, int,
-                                       int, int, int, int, const Coeffs<float>&, hipStream_t);
+                                       int, int, int, int, const Coeffs<float>&, const RiderCopy<float>*,
+                                       hipStream_t);
 template void launch_sparse_inv<double>(const double*, long long, const double*, long long, double*, long long,
-                                        int, int, int, int, int, const Coeffs<double>&, hipStream_t);
+                                        int, int, int, int, int, const Coeffs<double>&, const RiderCopy<double>*,
+                                        hipStream_t);
 
 }  // namespace dwt
