@@ -879,8 +879,14 @@ static RiderCopy<T> detail_copy(const T* src, long long sp, T* dst, long long dp
   return rc;
 }
 
-constexpr long long kCarrierBytes = 8ll << 20;
-constexpr int kBlockCarrierWeight = 3;
+#ifndef DWT_CARRIER_MIB
+#define DWT_CARRIER_MIB 8
+#endif
+#ifndef DWT_BLOCK_CARRIER_WEIGHT
+#define DWT_BLOCK_CARRIER_WEIGHT 3
+#endif
+constexpr long long kCarrierBytes = (long long)DWT_CARRIER_MIB << 20;
+constexpr int kBlockCarrierWeight = DWT_BLOCK_CARRIER_WEIGHT;
 
 // Assign rows [0, rows) of the copy to carriers by weight, in order; returns
 // false when no launch can carry any (all weights 0).
