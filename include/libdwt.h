@@ -466,17 +466,19 @@ int dwt_gpu_1d_batch(int wavelet, int direction, int precision, void *ptr, long 
 
 /* ---- batches of images of uneven sizes ------------------------------------
  * One call transforms many packed single-channel images, out of place, with
- * one launch per level for the whole batch.  Each image: src -> dst;
+ * one launch per level for the whole batch (and one for every image's small
+ * levels, dwt_gpu_batch_tail_set).  Each image: src -> dst;
  * forward: src is the FP32 image, dst receives its coefficients in Mallat
  * layout (the layout of dwt_cdf97_2f_s with size_o == size_i); inverse: src
  * holds the coefficients, dst receives the FP32 image.  The coefficients are
  * stored as FP32 (bit-identical to the single-image transform), FP16 or BF16;
  * the lifting arithmetic is FP32.  For 16-bit storage every subband is stored
- * multiplied by a power-of-two scale that the forward chooses from the
- * image's max |x| (so that |coefficient * scale| <= 2^14) and writes to
- * `scales` (3 per level: HL, LH, HH, then the final LL: 3 * levels + 1
- * floats in device memory); the inverse reads them from there.  Pitches are
- * in bytes, multiples of the element size.
+ * multiplied by a power-of-two scale that the forward writes to `scales` (3
+ * per level: HL, LH, HH, then the final LL: 3 * levels + 1 floats in device
+ * memory); the inverse reads them from there.  FP16: chosen from the image's
+ * max |x| so that |coefficient * scale| <= 2^14.  BF16: 1 (BF16 has FP32's
+ * exponent range, so a power-of-two scale would not change the stored
+ * mantissa).  Pitches are in bytes, multiples of the element size.
  * Images need width and height >= 2; levels as j_max of dwt_cdf97_2f_s with
  * decompose_one = 0 (forward: < 0 or too large means the maximum, written
  * back; inverse: the levels to undo). */

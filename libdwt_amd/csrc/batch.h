@@ -83,7 +83,8 @@ void launch_batch_maxabs(const BatchImage* imgs, const int* block_img, int nbloc
                          hipStream_t st);
 // gain: 3 per level (HL, LH, HH bound factors), then at 96 + J the bound of
 // the LL after J levels.
-void launch_batch_scales(const BatchImage* imgs, int n, const unsigned* amax, const float* gain,
+// unit: every scale 1 (no partials read).
+void launch_batch_scales(const BatchImage* imgs, int n, const unsigned* amax, const float* gain, int unit,
                          hipStream_t st);
 void analysis_gains(const Coeffs<double>& cf, double* gl, double* gh);
 // The tails of n images, one workgroup each (lds: dynamic LDS bytes, enough

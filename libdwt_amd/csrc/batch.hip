@@ -17,11 +17,16 @@
 // Storage: the coefficients (forward output, inverse input) are FP32, FP16 or
 // BF16; lifting arithmetic is FP32 throughout.  A 16-bit coefficient is
 // stored as round-to-nearest-even(c * s) for a per-subband power of two s
-// (exact to apply and to undo), chosen before the forward transform from the
-// image's max |x| and the filters' l1 gains so that |c * s| <= 2^14: no FP16
-// overflow, and the subband's largest coefficient uses the top of the FP16
-// exponent range.  Scales: 3 per level (HL, LH, HH) then one for the final
-// LL, in a float array per image that the inverse reads back.
+// (exact to apply and to undo).  FP16: s is chosen before the forward
+// transform from the image's max |x| and the filters' l1 gains so that
+// |c * s| <= 2^14: no FP16 overflow, and the subband's largest coefficient
+// uses the top of the FP16 exponent range.  BF16 has FP32's exponent range,
+// so a power-of-two scale changes only the exponent of what is stored, never
+// the rounded mantissa: round(c * s) / s == round(c) for every c that is not
+// an FP32 subnormal.  Its scales are 1, and the forward skips the pass over
+// the images that finding max |x| costs.  Scales: 3 per level (HL, LH, HH)
+// then one for the final LL, in a float array per image that the inverse
+// reads back.
 #include "block_body.h"
 #include "kernels.h"
 #include "batch.h"
@@ -290,13 +295,18 @@ batch_maxabs_kernel(const BatchImage* __restrict__ imgs, const int* __restrict__
 // Per-subband scales from max |x|: s = 2^floor(log2(2^14 / (gain * max|x|))).
 // One block per image: its threads reduce the image's per-block partials of
 // batch_maxabs_kernel, then one thread per subband computes its scale.
+// unit: every scale 1 (BF16 storage, below), no partials read.
 __global__ void __launch_bounds__(256)
 batch_scales_kernel(const BatchImage* __restrict__ imgs, int n, const unsigned* __restrict__ amax,
-                    const float* __restrict__ gain) {
+                    const float* __restrict__ gain, int unit) {
   __shared__ unsigned red[4];
   const int i = blockIdx.x;
   if (i >= n) return;
   const BatchImage& im = imgs[i];
+  if (unit) {
+    for (int k = threadIdx.x; k <= 3 * im.levels; k += 256) gstore_ptr(im.scale)[k] = 1.0f;
+    return;
+  }
   const int nb = (int)maxabs_blocks(im.W, im.H, im.sp);
   unsigned mb = 0;
   for (int b = threadIdx.x; b < nb; b += 256) mb = max(mb, amax[im.block0 + b]);
@@ -549,10 +559,10 @@ void launch_batch_maxabs(const BatchImage* imgs, const int* block_img, int nbloc
   hipLaunchKernelGGL(batch_maxabs_kernel, dim3(nblocks), dim3(256), 0, st, imgs, block_img, amax);
 }
 
-void launch_batch_scales(const BatchImage* imgs, int n, const unsigned* amax, const float* gain,
+void launch_batch_scales(const BatchImage* imgs, int n, const unsigned* amax, const float* gain, int unit,
                          hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(batch_scales_kernel, dim3(n), dim3(256), 0, st, imgs, n, amax, gain);
+  hipLaunchKernelGGL(batch_scales_kernel, dim3(n), dim3(256), 0, st, imgs, n, amax, gain, unit);
 }
 
 // l1 norms of the one-level analysis lowpass and highpass filters (libdwt's

@@ -218,7 +218,8 @@ void BatchTables::build(int wavelet_, bool fwd_, int storage_, const dwt_gpu_ima
     for (int i = n - 1; i >= 0; --i) {
       const dwt_gpu_image& im = images[i];
       imgs[i] = BatchImage{im.src, im.src_pitch / 4, im.scales, im.width, im.height, im.levels, (int)imap.size()};
-      const long long nb = maxabs_blocks(im.width, im.height, im.src_pitch / 4);
+      // BF16 needs no max |x| (its scales are 1, batch.hip)
+      const long long nb = storage == kStoreBF16 ? 0 : maxabs_blocks(im.width, im.height, im.src_pitch / 4);
       imap.insert(imap.end(), (size_t)nb, i);
     }
     double gl = 0, gh = 0;
@@ -257,7 +258,8 @@ void BatchTables::launch(char* dev, hipStream_t st) const {
     unsigned* amax = reinterpret_cast<unsigned*>(dev + off_amax);
     const BatchImage* im = reinterpret_cast<const BatchImage*>(dev + off_imgs);
     LAUNCH(launch_batch_maxabs(im, reinterpret_cast<const int*>(dev + off_imap), (int)imap.size(), amax, st);
-           launch_batch_scales(im, n, amax, reinterpret_cast<const float*>(dev + off_gain), st));
+           launch_batch_scales(im, n, amax, reinterpret_cast<const float*>(dev + off_gain),
+                               storage == kStoreBF16 ? 1 : 0, st));
   }
   auto tail = [&] {
     if (tails.empty()) return;

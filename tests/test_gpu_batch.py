@@ -7,8 +7,9 @@ Bars:
   * FP16 / BF16 storage: every stored coefficient equals the FP32 one scaled
     by its subband's power of two and rounded to nearest (so within half an
     ulp of the 16-bit format, relative 2^-11 / 2^-8, or the FP16 subnormal
-    spacing), the scales are powers of two with |c * s| <= 2^14, and the
-    reconstruction from 16-bit coefficients stays within the tolerance below.
+    spacing), the FP16 scales are powers of two with |c * s| <= 2^14, the BF16
+    scales are 1, and the reconstruction from 16-bit coefficients stays within
+    the tolerance below.
 """
 import sys
 
@@ -89,6 +90,8 @@ def check_16bit(imgs, wavelet, storage):
         assert s.shape == (3 * j + 1,)
         assert np.all(np.isfinite(s)) and np.all(s > 0)
         assert np.all(np.frexp(s)[0] == 0.5), "scales are powers of two"
+        if storage == "bf16":
+            assert np.all(s == 1.0), "BF16 keeps FP32's exponent range: scales 1"
         smap = np.full((H, W), np.nan)
         for lev in range(j):
             w, h = -(-W // 2 ** lev), -(-H // 2 ** lev)
@@ -101,7 +104,8 @@ def check_16bit(imgs, wavelet, storage):
         assert not np.isnan(smap).any()
         c = a.cpu().numpy().astype(np.float64)
         q = b.float().cpu().numpy().astype(np.float64)
-        assert np.abs(c * smap).max() <= 2.0 ** 14 * (1 + 2.0 ** -mant), x.shape
+        if storage == "f16":
+            assert np.abs(c * smap).max() <= 2.0 ** 14 * (1 + 2.0 ** -mant), x.shape
         # round to nearest: error of the scaled value <= half an ulp (or half
         # the FP16 subnormal spacing)
         err = np.abs(q - c * smap)
