@@ -92,7 +92,7 @@ void analysis_gains(const Coeffs<double>& cf, double* gl, double* gh);
 void launch_batch_tail(bool fwd, int storage, int n, size_t lds, const BatchTail* tails,
                        const Coeffs<float>& cf, hipStream_t st);
 // LDS bytes of a tail region of W x H samples: odd-pitch buffer, then the
-// scales and their inverses.
+// scales (or their inverses), then the line passes' dummy store slots.
 size_t batch_tail_lds(int W, int H);
 
 }  // namespace dwt

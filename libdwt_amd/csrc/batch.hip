@@ -428,15 +428,16 @@ batch_tail_fwd_kernel(const BatchTail* __restrict__ tails, Coeffs<float> cf) {
   Lds<float>* C = (Lds<float>*)blk_smem;  // generic -> LDS address-space cast
   const int pc = block_pitch(t.W);
   Lds<float>* S = C + pc * t.H;
+  Lds<float>* const dummy = S + 3 * 32 + 1;
   if constexpr (SK != kStoreF32)
     for (int k = threadIdx.x; k <= 3 * t.levels; k += kBlockThreads) S[k] = gload_ptr(t.scale)[k];
   block_copy<true>(C, pc, const_cast<float*>(static_cast<const float*>(t.src)), t.sp, t.W, t.H);
   __syncthreads();
   for (int lev = 0; lev < t.nlev; ++lev) {
     const int W = ceil_div_pow2(t.W, lev), H = ceil_div_pow2(t.H, lev);
-    block_pass_any<true, float, NS>(C, pc, 1, H, W, cf);  // rows
+    block_pass_any<true, float, NS>(C, pc, 1, H, W, cf, dummy);  // rows
     __syncthreads();
-    block_pass_any<true, float, NS>(C, 1, pc, W, H, cf);  // columns
+    block_pass_any<true, float, NS>(C, 1, pc, W, H, cf, dummy);  // columns
     __syncthreads();
   }
   if constexpr (SK == kStoreF32)
@@ -453,6 +454,7 @@ batch_tail_inv_kernel(const BatchTail* __restrict__ tails, Coeffs<float> cf) {
   Lds<float>* C = (Lds<float>*)blk_smem;  // generic -> LDS address-space cast
   const int pc = block_pitch(t.W);
   Lds<float>* S = C + pc * t.H;
+  Lds<float>* const dummy = S + 3 * 32 + 1;
   if constexpr (SK == kStoreF32) {
     block_copy<true>(C, pc, const_cast<float*>(static_cast<const float*>(t.src)), t.sp, t.W, t.H);
   } else {
@@ -463,9 +465,9 @@ batch_tail_inv_kernel(const BatchTail* __restrict__ tails, Coeffs<float> cf) {
   __syncthreads();
   for (int lev = t.nlev - 1; lev >= 0; --lev) {
     const int W = ceil_div_pow2(t.W, lev), H = ceil_div_pow2(t.H, lev);
-    block_pass_any<false, float, NS>(C, pc, 1, H, W, cf);  // rows
+    block_pass_any<false, float, NS>(C, pc, 1, H, W, cf, dummy);  // rows
     __syncthreads();
-    block_pass_any<false, float, NS>(C, 1, pc, W, H, cf);  // columns
+    block_pass_any<false, float, NS>(C, 1, pc, W, H, cf, dummy);  // columns
     __syncthreads();
   }
   block_copy<false>(C, pc, static_cast<float*>(t.dst), t.dp, t.W, t.H);
@@ -473,7 +475,7 @@ batch_tail_inv_kernel(const BatchTail* __restrict__ tails, Coeffs<float> cf) {
 
 // ---- host launchers --------------------------------------------------------------------
 size_t batch_tail_lds(int W, int H) {
-  return sizeof(float) * ((size_t)block_pitch(W) * H + 3 * 32 + 1);
+  return sizeof(float) * ((size_t)block_pitch(W) * H + 3 * 32 + 1 + kBlockDummy);
 }
 
 

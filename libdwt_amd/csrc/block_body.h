@@ -152,7 +152,10 @@ template <bool FWD, typename T, int NS, int PPL>
 struct LinePass {
   int ld_e[PPL], ld_o[PPL], st_e[PPL], st_o[PPL];
   int j0, n, nl, nh;
-  __device__ __forceinline__ void lift(Lds<T>* line, bool store, const Coeffs<T>& cf) const {
+  // A store the lane must not make goes to its own slot of the dummy area
+  // instead: an address select, where a conditional store would be a branch
+  // around it with exec-mask bookkeeping in every lifted line.
+  __device__ __forceinline__ void lift(Lds<T>* line, bool store, const Coeffs<T>& cf, Lds<T>* dummy) const {
     T e[PPL], o[PPL];
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
@@ -163,20 +166,21 @@ struct LinePass {
       lift_fwd<T, NS, PPL>(e, o, j0, n, cf);
     else
       lift_inv<T, NS, PPL>(e, o, j0, n, cf);
-    if (store) {
 #pragma unroll
-      for (int q = 0; q < PPL; ++q) {
-        if (j0 + q < nl) line[st_e[q]] = e[q];
-        if (j0 + q < nh) line[st_o[q]] = o[q];
-      }
+    for (int q = 0; q < PPL; ++q) {
+      Lds<T>* pe = store && j0 + q < nl ? line + st_e[q] : dummy;
+      Lds<T>* po = store && j0 + q < nh ? line + st_o[q] : dummy;
+      *pe = e[q];
+      *po = o[q];
     }
   }
 };
 
 template <bool FWD, typename T, int NS, int PPL>
 __device__ __forceinline__ void block_pass(Lds<T>* base, int lstep, int es, int nlines, int n, int hoff,
-                                           int lshift, const Coeffs<T>& cf) {
+                                           int lshift, const Coeffs<T>& cf, Lds<T>* dummy) {
   const int lane = threadIdx.x & (kWave - 1);
+  Lds<T>* const dl = dummy + lane;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   LinePass<FWD, T, NS, PPL> lp;
   lp.n = n;
@@ -205,8 +209,8 @@ __device__ __forceinline__ void block_pass(Lds<T>* base, int lstep, int es, int 
     const bool v0 = i0 + sub < nlines, v1 = i1 + sub < nlines;
     Lds<T>* l0 = base + i0 * lstep + (v0 ? sub_off : 0);
     Lds<T>* l1 = base + (i1 < nlines ? i1 : i0) * lstep + (v1 ? sub_off : 0);
-    lp.lift(l0, v0, cf);
-    lp.lift(l1, v1, cf);
+    lp.lift(l0, v0, cf, dl);
+    lp.lift(l1, v1, cf, dl);
   }
 }
 
@@ -216,21 +220,24 @@ __device__ __forceinline__ int block_ppl(int n) {
   return pairs <= kWave ? 1 : pairs <= 2 * kWave ? 2 : 4;
 }
 
-// hoff < 0: the packed frame's ceil(n/2).
+// dummy: kBlockDummy elements of LDS outside the region (one slot per lane)
+// that take the stores lanes must not make.  hoff < 0: the packed frame's
+// ceil(n/2).
+constexpr int kBlockDummy = kWave;
 template <bool FWD, typename T, int NS>
 __device__ __forceinline__ void block_pass_any(Lds<T>* base, int lstep, int es, int nlines, int n,
-                                               const Coeffs<T>& cf, int hoff = -1) {
+                                               const Coeffs<T>& cf, Lds<T>* dummy, int hoff = -1) {
   if (hoff < 0) hoff = (n + 1) >> 1;
   const int ppl = block_ppl(n);
   if (ppl == 1) {
     const int pairs = (n + 1) >> 1;
     int lshift = 0;
     while ((1 << lshift) < pairs) ++lshift;
-    block_pass<FWD, T, NS, 1>(base, lstep, es, nlines, n, hoff, lshift, cf);
+    block_pass<FWD, T, NS, 1>(base, lstep, es, nlines, n, hoff, lshift, cf, dummy);
   } else if (ppl == 2) {
-    block_pass<FWD, T, NS, 2>(base, lstep, es, nlines, n, hoff, 6, cf);
+    block_pass<FWD, T, NS, 2>(base, lstep, es, nlines, n, hoff, 6, cf, dummy);
   } else {
-    block_pass<FWD, T, NS, 4>(base, lstep, es, nlines, n, hoff, 6, cf);
+    block_pass<FWD, T, NS, 4>(base, lstep, es, nlines, n, hoff, 6, cf, dummy);
   }
 }
 

@@ -18,13 +18,14 @@ block_fwd_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, l
   extern __shared__ __align__(16) unsigned char blk_smem[];
   Lds<T>* C = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
   const int pc = block_pitch(W0);
+  Lds<T>* const dummy = C + pc * H0;
   block_copy<true>(C, pc, const_cast<T*>(src), sp, W0, H0);
   __syncthreads();
   for (int lev = 0; lev < nlev; ++lev) {
     const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
-    block_pass_any<true, T, NS>(C, pc, 1, H, W, cf);  // rows
+    block_pass_any<true, T, NS>(C, pc, 1, H, W, cf, dummy);  // rows
     __syncthreads();
-    block_pass_any<true, T, NS>(C, 1, pc, W, H, cf);  // columns
+    block_pass_any<true, T, NS>(C, 1, pc, W, H, cf, dummy);  // columns
     __syncthreads();
   }
   block_copy<false>(C, pc, dst, dp, W0, H0);
@@ -41,13 +42,14 @@ block_inv_kernel(const T* __restrict__ src, long long sp, T* __restrict__ dst, l
   extern __shared__ __align__(16) unsigned char blk_smem[];
   Lds<T>* C = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
   const int pc = block_pitch(W0);
+  Lds<T>* const dummy = C + pc * H0;
   block_copy<true>(C, pc, const_cast<T*>(src), sp, W0, H0);
   __syncthreads();
   for (int lev = nlev - 1; lev >= 0; --lev) {
     const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
-    block_pass_any<false, T, NS>(C, pc, 1, H, W, cf);  // rows
+    block_pass_any<false, T, NS>(C, pc, 1, H, W, cf, dummy);  // rows
     __syncthreads();
-    block_pass_any<false, T, NS>(C, 1, pc, W, H, cf);  // columns
+    block_pass_any<false, T, NS>(C, 1, pc, W, H, cf, dummy);  // columns
     __syncthreads();
   }
   block_copy<false>(C, pc, dst, dp, W0, H0);
@@ -65,14 +67,15 @@ block_sparse_fwd_kernel(T* __restrict__ a, long long ap, int W0, int H0, int Wi0
   extern __shared__ __align__(16) unsigned char blk_smem[];
   Lds<T>* C = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
   const int pc = block_pitch(W0);
+  Lds<T>* const dummy = C + pc * H0;
   block_copy<true>(C, pc, a, ap, W0, H0);
   __syncthreads();
   for (int lev = 0; lev < nlev; ++lev) {
     const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
     const int Wi = ceil_div_pow2(Wi0, lev), Hi = ceil_div_pow2(Hi0, lev);
-    block_pass_any<true, T, NS>(C, pc, 1, H, Wi, cf, (W + 1) >> 1);  // rows y < H, length Wi
+    block_pass_any<true, T, NS>(C, pc, 1, H, Wi, cf, dummy, (W + 1) >> 1);  // rows y < H, length Wi
     __syncthreads();
-    block_pass_any<true, T, NS>(C, 1, pc, W, Hi, cf, (H + 1) >> 1);  // columns x < W, length Hi
+    block_pass_any<true, T, NS>(C, 1, pc, W, Hi, cf, dummy, (H + 1) >> 1);  // columns x < W, length Hi
     __syncthreads();
     if (zp) {
       block_zero_pad<true, T>(C, pc, W, H, Wi, Hi);
@@ -89,6 +92,7 @@ block_sparse_inv_kernel(T* __restrict__ a, long long ap, int W0, int H0, int Wi0
   extern __shared__ __align__(16) unsigned char blk_smem[];
   Lds<T>* C = (Lds<T>*)blk_smem;  // generic -> LDS address-space cast
   const int pc = block_pitch(W0);
+  Lds<T>* const dummy = C + pc * H0;
   block_copy<true>(C, pc, a, ap, W0, H0);
   __syncthreads();
   for (int lev = nlev - 1; lev >= 0; --lev) {
@@ -96,9 +100,9 @@ block_sparse_inv_kernel(T* __restrict__ a, long long ap, int W0, int H0, int Wi0
     // level's outer size
     const int W = ceil_div_pow2(W0, lev), H = ceil_div_pow2(H0, lev);
     const int Wi = ceil_div_pow2(Wi0, lev), Hi = ceil_div_pow2(Hi0, lev);
-    block_pass_any<false, T, NS>(C, pc, 1, H, Wi, cf, (W + 1) >> 1);  // rows y < H
+    block_pass_any<false, T, NS>(C, pc, 1, H, Wi, cf, dummy, (W + 1) >> 1);  // rows y < H
     __syncthreads();
-    block_pass_any<false, T, NS>(C, 1, pc, W, Hi, cf, (H + 1) >> 1);  // columns x < W
+    block_pass_any<false, T, NS>(C, 1, pc, W, Hi, cf, dummy, (H + 1) >> 1);  // columns x < W
     __syncthreads();
     if (zp) {
       block_zero_pad<false, T>(C, pc, W, H, Wi, Hi);
@@ -125,7 +129,7 @@ void tail_config(long long max_region_bytes, int) {
 }
 
 static long long block_lds_bytes(long long W0, long long H0, int elem) {
-  return elem * (long long)block_pitch((int)W0) * H0;
+  return elem * ((long long)block_pitch((int)W0) * H0 + kBlockDummy);
 }
 
 int tail_start(int W, int H, int L, int elem) {
