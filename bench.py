@@ -563,6 +563,40 @@ def bench_lines(args, torch, libdwt_amd, workloads):
     print(json.dumps(out), flush=True)
 
 
+def measure_host_dropin(torch, libdwt_amd, img: np.ndarray, wavelet: str, J: int, steps: int = 5) -> dict:
+    """The drop-in drivers (dwt_<wavelet>_2f_s / _2i_s, synchronous) on the
+    image in host memory, as a reference caller holds it: pageable and pinned
+    (page-locked) buffers.  The rate includes the host-device transfers; it is
+    never the headline value."""
+    import ctypes
+    L = libdwt_amd.lib()
+    f = getattr(L, f"dwt_{wavelet}_2f_s")
+    i = getattr(L, f"dwt_{wavelet}_2i_s")
+    ci = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ci, ci, ci, ci, ci, ci, ctypes.POINTER(ci), ci, ci]
+    i.argtypes = [ctypes.c_void_p, ci, ci, ci, ci, ci, ci, ci, ci, ci]
+    f.restype = i.restype = None
+    H, W = img.shape
+    out = {}
+    pinned = torch.from_numpy(img.copy()).pin_memory()
+    for name, buf in (("pageable", img.copy()), ("pinned", pinned)):
+        ptr = buf.ctypes.data if name == "pageable" else buf.data_ptr()
+
+        def step():
+            jj = ci(J)
+            f(ptr, 4 * W, 4, W, H, W, H, ctypes.byref(jj), 0, 0)
+            i(ptr, 4 * W, 4, W, H, W, H, jj.value, 0, 0)
+        step()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        back = buf if name == "pageable" else buf.numpy()
+        out[name] = {"ms_per_step": round(ms, 3), "Mpix_s": round(W * H / (ms * 1e-3) / 1e6, 1),
+                     "max_abs_roundtrip_err": float(np.abs(back - img).max())}
+    return out
+
+
 def measure_lines_dropin(torch, libdwt_amd, workloads, wavelet: str, count: int = 64, N: int = 8192) -> dict:
     """The reference's own caller loop for a full-depth 1D transform
     (src/libdwt.h:219-348: one dwt_cdf97_f_ex_stride_s call per signal and
@@ -873,6 +907,7 @@ def main():
                                  "coef_max_abs_err_vs_fp64": b["modes"]["f32"]["coef_max_abs_err_vs_fp64"]},
         }
         out["lines_dropin"] = measure_lines_dropin(torch, libdwt_amd, workloads, wv)
+        out["host_dropin"] = measure_host_dropin(torch, libdwt_amd, img, wv, J)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(W, H, wv, J, args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -157,6 +157,29 @@ struct Scratch {
   // 8: packed copy of a restaged frame
   DevBuf buf[9];
   void* cmp_host = nullptr;  // pinned copy of a compare result
+  // staging of pageable host frames: two pinned chunks, each with an event
+  // marking the end of the last copy engine transfer that used it
+  void* pin[2] = {nullptr, nullptr};
+  size_t pin_bytes = 0;
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
+  void ensure_pin(size_t bytes) {
+    if (pin_bytes >= bytes) return;
+    for (int i = 0; i < 2; ++i) {
+      if (pin[i]) {
+        HIPCHK(hipEventSynchronize(pin_ev[i]));
+        hipHostFree(pin[i]);
+        pin[i] = nullptr;
+      }
+      if (hipHostMalloc(&pin[i], bytes, hipHostMallocDefault) != hipSuccess) {
+        hipGetLastError();
+        pin[i] = nullptr;
+        pin_bytes = 0;
+        fail(DWT_GPU_EMEMORY, "pinned staging of %zu bytes failed", bytes);
+      }
+      if (!pin_ev[i]) HIPCHK(hipEventCreateWithFlags(&pin_ev[i], hipEventDisableTiming));
+    }
+    pin_bytes = bytes;
+  }
   // pyramid state: plans (most recent first), and the sync words
   // ctrl[0..1], err[2], then per-segment counters
   std::vector<PyrPlan*> plans;
@@ -231,6 +254,13 @@ struct Scratch {
     err_host = nullptr;
     if (cmp_host) hipHostFree(cmp_host);
     cmp_host = nullptr;
+    for (int i = 0; i < 2; ++i) {
+      if (pin[i]) hipHostFree(pin[i]);
+      if (pin_ev[i]) hipEventDestroy(pin_ev[i]);
+      pin[i] = nullptr;
+      pin_ev[i] = nullptr;
+    }
+    pin_bytes = 0;
     if (trace) hipFree(trace);
     trace = nullptr;
     trace_n = trace_items = 0;
@@ -1512,19 +1542,75 @@ static void* device_address(const void* p) {
   }
 }
 
-// Gather the outer frame of a host image into a packed host vector and back.
+// Pageable host frames move through two pinned chunks of about kStageChunk
+// bytes, in whole rows: upload, the host copies rows of chunk c into one
+// pinned chunk while the copy engine uploads chunk c - 1 from the other;
+// download, the copy engine fills one chunk while the host copies the other
+// out.  (A pageable hipMemcpy goes through the runtime's own small staging
+// and moved a 4096^2 FP32 frame at under 2 GB/s.)  Rows are contiguous when
+// stride_y is the element size; any other element stride is gathered and
+// scattered sample by sample on the host.
+constexpr size_t kStageChunk = 8u << 20;
+
 template <typename T>
-static void host_gather(std::vector<T>& v, const char* ptr, long long sx, long long sy, int Wo,
-                        int Ho) {
-  v.resize((size_t)Wo * Ho);
-  for (int y = 0; y < Ho; ++y)
-    for (int x = 0; x < Wo; ++x) std::memcpy(&v[(size_t)y * Wo + x], ptr + y * sx + x * sy, sizeof(T));
+static void host_rows_in(char* dst, const char* src, long long sx, long long sy, int Wo, int y0, int n) {
+  const size_t pitch = (size_t)Wo * sizeof(T);
+  for (int r = 0; r < n; ++r) {
+    const char* s = src + (long long)(y0 + r) * sx;
+    char* d = dst + (size_t)r * pitch;
+    if (sy == (long long)sizeof(T))
+      std::memcpy(d, s, pitch);
+    else
+      for (int x = 0; x < Wo; ++x) std::memcpy(d + (size_t)x * sizeof(T), s + (long long)x * sy, sizeof(T));
+  }
 }
 template <typename T>
-static void host_scatter(const std::vector<T>& v, char* ptr, long long sx, long long sy, int Wo,
-                         int Ho) {
-  for (int y = 0; y < Ho; ++y)
-    for (int x = 0; x < Wo; ++x) std::memcpy(ptr + y * sx + x * sy, &v[(size_t)y * Wo + x], sizeof(T));
+static void host_rows_out(char* dst, const char* src, long long sx, long long sy, int Wo, int y0, int n) {
+  const size_t pitch = (size_t)Wo * sizeof(T);
+  for (int r = 0; r < n; ++r) {
+    char* d = dst + (long long)(y0 + r) * sx;
+    const char* s = src + (size_t)r * pitch;
+    if (sy == (long long)sizeof(T))
+      std::memcpy(d, s, pitch);
+    else
+      for (int x = 0; x < Wo; ++x) std::memcpy(d + (long long)x * sy, s + (size_t)x * sizeof(T), sizeof(T));
+  }
+}
+
+template <typename T>
+static void stage_in(Scratch& sc, const char* host, long long sx, long long sy, int Wo, int Ho, char* dev,
+                     hipStream_t st) {
+  const size_t pitch = (size_t)Wo * sizeof(T);
+  const int R = (int)std::max<size_t>(1, kStageChunk / pitch);  // rows per chunk
+  sc.ensure_pin((size_t)R * pitch);
+  for (int c = 0, y = 0; y < Ho; ++c, y += R) {
+    const int k = c & 1, n = std::min(R, Ho - y);
+    if (c >= 2) HIPCHK(hipEventSynchronize(sc.pin_ev[k]));  // its last upload is done
+    host_rows_in<T>((char*)sc.pin[k], host, sx, sy, Wo, y, n);
+    HIPCHK(hipMemcpyAsync(dev + (size_t)y * pitch, sc.pin[k], (size_t)n * pitch, hipMemcpyHostToDevice, st));
+    HIPCHK(hipEventRecord(sc.pin_ev[k], st));
+  }
+}
+
+template <typename T>
+static void stage_out(Scratch& sc, const char* dev, char* host, long long sx, long long sy, int Wo, int Ho,
+                      hipStream_t st) {
+  const size_t pitch = (size_t)Wo * sizeof(T);
+  const int R = (int)std::max<size_t>(1, kStageChunk / pitch);
+  sc.ensure_pin((size_t)R * pitch);
+  const int nc = (Ho + R - 1) / R;
+  auto issue = [&](int c) {
+    const int k = c & 1, y = c * R, n = std::min(R, Ho - y);
+    HIPCHK(hipMemcpyAsync(sc.pin[k], dev + (size_t)y * pitch, (size_t)n * pitch, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(sc.pin_ev[k], st));
+  };
+  for (int c = 0; c < nc && c < 2; ++c) issue(c);
+  for (int c = 0; c < nc; ++c) {
+    const int k = c & 1, y = c * R, n = std::min(R, Ho - y);
+    HIPCHK(hipEventSynchronize(sc.pin_ev[k]));
+    host_rows_out<T>(host, (const char*)sc.pin[k], sx, sy, Wo, y, n);
+    if (c + 2 < nc) issue(c + 2);
+  }
 }
 
 template <typename T>
@@ -1556,27 +1642,13 @@ static void run2d(bool fwd, void* ptr, int sx, int sy, int Wo, int Ho, int Wi, i
   Scratch& sc = scratch_for(st);
   const size_t pitch = (size_t)Wo * sizeof(T);
   char* dev = (char*)sc.get(4, pitch * (size_t)Ho);
-  const bool rows = sy == (int)sizeof(T) && sx > 0 && (size_t)sx >= pitch;
-  std::vector<T> host;
-  if (rows) {
-    HIPCHK(hipMemcpy2DAsync(dev, pitch, ptr, sx, pitch, Ho, hipMemcpyHostToDevice, st));
-  } else {
-    host_gather<T>(host, (const char*)ptr, sx, sy, Wo, Ho);
-    HIPCHK(hipMemcpyAsync(dev, host.data(), pitch * Ho, hipMemcpyHostToDevice, st));
-  }
+  stage_in<T>(sc, (const char*)ptr, sx, sy, Wo, Ho, dev, st);
   if (fwd)
     fwd2_device<T>(dev, (long long)pitch, sizeof(T), Wo, Ho, Wi, Hi, jp, d1, zp, w, st);
   else
     inv2_device<T>(dev, (long long)pitch, sizeof(T), Wo, Ho, Wi, Hi, *jp, d1, zp, w, st);
-  if (rows) {
-    HIPCHK(hipMemcpy2DAsync(ptr, sx, dev, pitch, pitch, Ho, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-  } else {
-    HIPCHK(hipMemcpyAsync(host.data(), dev, pitch * Ho, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
-  }
+  stage_out<T>(sc, dev, (char*)ptr, sx, sy, Wo, Ho, st);
   if (pyr_failed(sc, st)) fail(DWT_GPU_EHIP, "pyramid dependency wait timed out");
-  if (!rows) host_scatter<T>(host, (char*)ptr, sx, sy, Wo, Ho);
 }
 
 // One level of one strided line (src/libdwt.c:5512-5603, 5779-5870).
