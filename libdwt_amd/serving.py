@@ -49,27 +49,13 @@ class Server:
         self.h = C.c_void_p()
         libdwt_amd._check(self.L.dwt_gpu_server_create(budget_bytes, max_batch, streams,
                                                        C.byref(self.h)))
-        self._req = Request()
         self._id = C.c_longlong()
 
     def submit(self, wavelet: str, forward: bool, src, dst, levels: int = -1,
                storage: str = "f32", scales=None) -> int:
         """src / dst: 2D CUDA tensors (image float32 / coefficients in the
         storage dtype, by direction); returns the request id."""
-        r = self._req
-        r.wavelet = libdwt_amd.WAVELET_IDS[wavelet]
-        r.direction = libdwt_amd.FORWARD if forward else libdwt_amd.INVERSE
-        r.storage = libdwt_amd.STORAGE[storage]
-        im = r.image
-        H, W = src.shape
-        im.src = src.data_ptr()
-        im.dst = dst.data_ptr()
-        im.src_pitch = src.stride(0) * src.element_size()
-        im.dst_pitch = dst.stride(0) * dst.element_size()
-        im.width = W
-        im.height = H
-        im.levels = levels
-        im.scales = scales.data_ptr() if scales is not None else None
+        r = self.request(wavelet, forward, src, dst, levels, storage, scales)
         libdwt_amd._check(self.L.dwt_gpu_server_submit(self.h, C.byref(r), C.byref(self._id)))
         return self._id.value
 
