@@ -54,6 +54,21 @@ struct BatchTail {
 // plus the scales stays within one workgroup's 160 KiB.
 constexpr long long kBatchTailMax = 128 * 1024;
 
+// Tiles of an item to its workgroups of four waves.  DWT_BATCH_PACK 1: the
+// item's nstrips x nseg tiles, numbered row-major, four to a workgroup across
+// segment boundaries, so a narrow image (one or two strips) fills its
+// workgroups' waves; 0: each workgroup takes four strips of one segment.
+#ifndef DWT_BATCH_PACK
+#define DWT_BATCH_PACK 1
+#endif
+__host__ __device__ inline int batch_item_blocks(int nstrips, int nseg) {
+#if DWT_BATCH_PACK
+  return (nstrips * nseg + 3) / 4;
+#else
+  return ((nstrips + 3) / 4) * nseg;
+#endif
+}
+
 // One image of a batch for the max |x| pass and the scale computation.
 struct BatchImage {
   const void* src;

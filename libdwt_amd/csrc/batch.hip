@@ -111,6 +111,22 @@ struct OutRt {
 };
 
 // ---- level kernels ------------------------------------------------------------------
+// This wave's tile (segment, strip) of an item (batch_item_blocks, batch.h).
+__device__ __forceinline__ bool batch_tile(int local, const BatchItem& e, int& s, int& strip) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  static_assert(kWavesPerBlock == 4, "batch_item_blocks deals four tiles per workgroup");
+#if DWT_BATCH_PACK
+  const int t = local * kWavesPerBlock + wave;
+  s = t / e.nstrips;
+  strip = t - s * e.nstrips;
+  return s < e.nseg;
+#else
+  s = local / e.ngroups;
+  strip = (local - s * e.ngroups) * kWavesPerBlock + wave;
+  return strip < e.nstrips;
+#endif
+}
+
 // Workgroup order of the batched level kernels: the hardware deals
 // consecutive workgroups to the 8 XCDs round-robin; here each XCD takes
 // kBatchBand consecutive logical workgroups in turn (the tail of a grid that
@@ -132,9 +148,8 @@ batch_fwd_kernel(const BatchItem* __restrict__ items, const int* __restrict__ bl
   const int b = batch_band_remap(blockIdx.x, gridDim.x);
   const BatchItem& e = items[block_item[b]];
   const int local = b - e.block0;
-  const int s = local / e.ngroups, g = local - s * e.ngroups;
-  const int strip = g * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (strip >= e.nstrips) return;
+  int s, strip;
+  if (!batch_tile(local, e, s, strip)) return;
   const RowsRt rows{{static_cast<const float*>(e.src), e.sp, e.H},
                     __builtin_amdgcn_readfirstlane(e.flags & kItemVec) != 0};
   if constexpr (SK == kStoreF32) {
@@ -163,9 +178,8 @@ batch_inv_kernel(const BatchItem* __restrict__ items, const int* __restrict__ bl
   const int b = batch_band_remap(blockIdx.x, gridDim.x);
   const BatchItem& e = items[block_item[b]];
   const int local = b - e.block0;
-  const int s = local / e.ngroups, g = local - s * e.ngroups;
-  const int strip = g * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (strip >= e.nstrips) return;
+  int s, strip;
+  if (!batch_tile(local, e, s, strip)) return;
   const OutRt out{{static_cast<float*>(e.out), e.op},
                   {static_cast<float*>(e.out), e.op},
                   __builtin_amdgcn_readfirstlane(e.flags & kItemVec) != 0};

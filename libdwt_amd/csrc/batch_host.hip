@@ -174,7 +174,7 @@ void BatchTables::build(int wavelet_, bool fwd_, int storage_, const dwt_gpu_ima
       e.ngroups = (e.nstrips + 3) / 4;
       e.nseg = (e.H + lv.S - 1) / lv.S;
       e.block0 = blocks;
-      const int nb = e.ngroups * e.nseg;
+      const int nb = batch_item_blocks(e.nstrips, e.nseg);
       const int idx = (int)(items.size() - lv.items_off);
       bmap.insert(bmap.end(), nb, idx);
       blocks += nb;
