@@ -29,6 +29,16 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def rnd(x, nd):
+    """round(x, nd), None passing through."""
+    return None if x is None else round(x, nd)
+
+
+def gbs(nbytes: float, us):
+    """GB/s of nbytes moved in us microseconds (None when us is None)."""
+    return nbytes / (us * 1e-6) / 1e9 if us else None
+
+
 STEP_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_step_4096x4096_cdf97_J8.md")
 
 
@@ -744,39 +754,55 @@ def main():
     rec_err = float((t.cpu() - torch.from_numpy(img)).abs().max())
 
     # cold steps: the image and scratch (~150 MB) fit the 256 MB Infinity
-    # Cache, so back-to-back steps are partly served from it; here each step
-    # runs alone after writing a 512 MiB buffer (untimed) to evict them
-    cold, cold_l0_us = [], None
+    # Cache, so back-to-back steps are partly served from it.  Here each step
+    # runs alone after a 512 MiB buffer has gone through the cache (untimed):
+    #   clean: a reduction reads the buffer, leaving the cache full of clean
+    #          lines of it; the step pays for its own misses only;
+    #   dirty: a fill writes it, leaving up to the cache's size of dirty lines
+    #          that the step's traffic must first write back to HBM.
+    # A plain copy of the level-0 bytes is timed the same way (clean) as the
+    # cold ceiling.
+    cold_runs = {}
+    cold_copy_us = None
     if args.cold_steps > 0:
         scrub = torch.empty(512 << 18, device=dev)
+        sink = torch.empty((), device=dev)
+        with torch.cuda.stream(stream):
+            scrub.fill_(1.0)
+        scrubs = {"clean": lambda: torch.sum(scrub, dim=0, out=sink),
+                  "dirty": lambda: scrub.fill_(1.0)}
 
-        def cold_steps():
-            for _ in range(args.cold_steps):
+        def timed_after(scrub_op, body, n):
+            times = []
+            for _ in range(n):
                 with torch.cuda.stream(stream):
-                    scrub.fill_(1.0)
+                    scrub_op()
                 c0 = torch.cuda.Event(enable_timing=True)
                 c1 = torch.cuda.Event(enable_timing=True)
-                c0.record(stream)
-                step()
-                c1.record(stream)
+                with torch.cuda.stream(stream):
+                    c0.record(stream)
+                    body()
+                    c1.record(stream)
                 torch.cuda.synchronize()
-                cold.append(c0.elapsed_time(c1))
+                times.append(c0.elapsed_time(c1))
+            return sorted(times)
 
-        cold_steps()
-        cold.sort()
-        step_times = cold
-        # again with HIP events around every launch: the level-0 kernel alone
-        # after the scrub
-        cold = []
-        libdwt_amd.profile_reset()
-        libdwt_amd.profile_enable(True)
-        cold_steps()
-        libdwt_amd.profile_enable(False)
-        n0, ms0 = libdwt_amd.profile_read(0, 0)
-        cold_l0_us = 1000 * ms0 / n0 if n0 else None
-        libdwt_amd.profile_reset()
-        cold = step_times
-        del scrub
+        for kind, op in scrubs.items():
+            step_times = timed_after(op, step, args.cold_steps)
+            # again with HIP events around every launch: the level-0 kernel
+            # alone after the scrub
+            libdwt_amd.profile_reset()
+            libdwt_amd.profile_enable(True)
+            timed_after(op, step, args.cold_steps)
+            libdwt_amd.profile_enable(False)
+            n0, ms0 = libdwt_amd.profile_read(0, 0)
+            libdwt_amd.profile_reset()
+            cold_runs[kind] = (step_times, 1000 * ms0 / n0 if n0 else None)
+        ca = torch.empty(W * H, device=dev).fill_(1.0)
+        cb_ = torch.empty_like(ca)
+        copies = timed_after(scrubs["clean"], lambda: cb_.copy_(ca), args.cold_steps)
+        cold_copy_us = 1000 * copies[len(copies) // 2]
+        del scrub, ca, cb_
 
     # per-kernel timing pass (same steps, HIP events around every launch)
     libdwt_amd.profile_reset()
@@ -832,6 +858,8 @@ def main():
     achieved = alg_bytes / (lv0_us * 1e-6) / 1e9 if lv0_us else None
 
     mpix = W * H / (ms_per_step * 1e-3) / 1e6
+    cold_l0_us = cold_runs["clean"][1] if "clean" in cold_runs else None
+    cold_dirty_l0_us = cold_runs["dirty"][1] if "dirty" in cold_runs else None
     alg_step = workloads.algorithmic_bytes(W, H, J) * 2
     headline = (W, H, J, wv) == (4096, 4096, 8, "cdf97")
     metric = ("Mpix/s + %HBM peak, 4096\u00b2 CDF9/7 J=8 fwd+inv; batched p50/p99 latency" if headline else
@@ -866,23 +894,32 @@ def main():
             "avg_launch_us": round(lv0_us, 3) if lv0_us else None,
             "avg_launch_us_single_events": lv0_single,
             "launches_timed": n_rep,
-            # the same kernel right after a 512 MiB scrub (nothing of the
-            # image in the 256 MB Infinity Cache), one launch between events,
-            # beside the warm single-launch figure timed the same way
-            "cold_single_launch_us": round(cold_l0_us, 3) if cold_l0_us else None,
-            "achieved_cold": round(alg_bytes / (cold_l0_us * 1e-6) / 1e9, 1) if cold_l0_us else None,
-            "frac_cold": round(alg_bytes / (cold_l0_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4) if cold_l0_us else None,
+            # the same kernel right after a 512 MiB buffer went through the
+            # cache (nothing of the image in the 256 MB Infinity Cache), one
+            # launch between events, beside the warm single-launch figure
+            # timed the same way: clean (the buffer read) and dirty (written)
+            "cold_single_launch_us": rnd(cold_l0_us, 3),
+            "achieved_cold": rnd(gbs(alg_bytes, cold_l0_us), 1),
+            "frac_cold": rnd(gbs(alg_bytes, cold_l0_us) / HBM_PEAK_GBS if cold_l0_us else None, 4),
+            "cold_dirty_single_launch_us": rnd(cold_dirty_l0_us, 3),
+            "frac_cold_dirty": rnd(gbs(alg_bytes, cold_dirty_l0_us) / HBM_PEAK_GBS if cold_dirty_l0_us else None, 4),
             "hbm_frac_step": round(alg_step / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "step_algorithmic_bytes": alg_step,
             "step_traffic": step_traffic,
             "step_traffic_over_algorithmic": round(step_traffic / alg_step, 3) if step_traffic else None,
             "copy_ceiling": {"gbs": round(copy_gbs, 1), "avg_us": round(copy_us, 3),
                              "bytes": 8 * W * H,
-                             "frac_of_copy": round(achieved / copy_gbs, 4) if achieved else None},
+                             "frac_of_copy": round(achieved / copy_gbs, 4) if achieved else None,
+                             "cold_us": rnd(cold_copy_us, 3),
+                             "cold_gbs": rnd(gbs(8 * W * H, cold_copy_us), 1),
+                             "frac_of_copy_cold": rnd(cold_copy_us / cold_l0_us
+                                                      if cold_copy_us and cold_l0_us else None, 4)},
         },
-        "cold": ({"ms_per_step_median": round(cold[len(cold) // 2], 5),
-                  "value": round(W * H / (cold[len(cold) // 2] * 1e-3) / 1e6, 3),
-                  "steps": len(cold), "scrub_mib": 512} if cold else None),
+        "cold": ({kind: {"ms_per_step_median": round(t[len(t) // 2], 5),
+                         "value": round(W * H / (t[len(t) // 2] * 1e-3) / 1e6, 3),
+                         "level0_fwd_us": rnd(l0, 3)}
+                  for kind, (t, l0) in cold_runs.items()} | {"steps": args.cold_steps, "scrub_mib": 512}
+                 if cold_runs else None),
         "schedule": "pyramid" if args.schedule == 1 else "per-level",
         "kernels": kernels,
         "wall_s": round(wall, 4),
