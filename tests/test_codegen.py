@@ -12,6 +12,7 @@ import os
 import re
 import subprocess
 import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 import pytest
 
@@ -22,17 +23,20 @@ SOURCES = hipbuild.SOURCES
 
 @pytest.fixture(scope="module")
 def asm():
-    out = {}
+    """Device assembly of every source, compiled in parallel."""
     with tempfile.TemporaryDirectory() as d:
-        for src in SOURCES:
+        def compile_one(src):
             s = os.path.join(d, src + ".s")
             cmd = [hipbuild.HIPCC, *hipbuild.FLAGS, "--cuda-device-only", "-S",
                    os.path.join(hipbuild.CSRC, src), "-o", s]
             r = subprocess.run(cmd, capture_output=True, text=True)
             assert r.returncode == 0, r.stderr[-2000:]
             with open(s) as fh:
-                out[src] = fh.read()
-    return out
+                return src, fh.read()
+
+        workers = max(1, min(len(SOURCES), os.cpu_count() or 1))
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            return dict(ex.map(compile_one, SOURCES))
 
 
 def test_no_flat_memory_instructions(asm):
