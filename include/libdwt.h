@@ -565,6 +565,15 @@ int dwt_gpu_col_plan(int forward, int n, int nsteps, int *stages, int *top_h_row
 int dwt_gpu_col_segment_rows(int forward, int n, int nsteps, int seg, int *reads, int max_reads,
                              int *nreads, int *writes, int max_writes, int *nwrites);
 
+/* Test support (no counterpart in the reference): builds the tables of an FP32
+ * batch (dwt_gpu_batch_plan_create's descriptors; the buffers are never
+ * touched, so any distinct non-null pointers do) on the host and replays every
+ * level launch's workgroup -> (image, tile) mapping of the batched level
+ * kernels; returns DWT_GPU_OK when each tile of every image and level is
+ * computed by exactly one wave, an error code (message in
+ * dwt_gpu_last_error) otherwise. */
+int dwt_gpu_batch_order_check(int wavelet, int direction, const dwt_gpu_image *images, int n);
+
 /* Bytes of device scratch the calling thread currently holds for `stream`.
  * Scratch is kept per (host thread, stream, device) and reused by later calls;
  * a thread's sets are freed when the thread exits (except the thread that

@@ -112,42 +112,28 @@ struct OutRt {
 
 // ---- level kernels ------------------------------------------------------------------
 // This wave's tile (segment, strip) of an item (batch_item_blocks, batch.h).
+static_assert(kWavesPerBlock == 4, "batch_item_blocks and batch_tile_of deal four tiles per workgroup");
 __device__ __forceinline__ bool batch_tile(int local, const BatchItem& e, int& s, int& strip) {
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  static_assert(kWavesPerBlock == 4, "batch_item_blocks deals four tiles per workgroup");
-#if DWT_BATCH_PACK
-  const int t = local * kWavesPerBlock + wave;
-  s = t / e.nstrips;
-  strip = t - s * e.nstrips;
-  return s < e.nseg;
-#else
-  s = local / e.ngroups;
-  strip = (local - s * e.ngroups) * kWavesPerBlock + wave;
-  return strip < e.nstrips;
-#endif
+  return batch_tile_of(local, __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), e, s, strip);
 }
 
-// Workgroup order of the batched level kernels: the hardware deals
-// consecutive workgroups to the 8 XCDs round-robin; here each XCD takes
-// kBatchBand consecutive logical workgroups in turn (the tail of a grid that
-// is not a whole number of rounds keeps the plain order).  Measured on 256
-// uneven images against plain order (1), 4, 16, 32, 64 and 128 and one
-// contiguous run per XCD: 8 was fastest (FP32 step 0.645 vs 0.680 ms).
-constexpr int kBatchBand = 8;
-__device__ inline int batch_band_remap(int b, int nb) {
-  const int full = (nb / (8 * kBatchBand)) * (8 * kBatchBand);
-  if (b >= full) return b;
-  const int x = b & 7, idx = b >> 3;
-  return (x + 8 * (idx / kBatchBand)) * kBatchBand + idx % kBatchBand;
+__device__ __forceinline__ bool batch_block(const BatchItem* __restrict__ items,
+                                            const int* __restrict__ block_item, const BatchItem*& e,
+                                            int& local) {
+  int it;
+  if (!batch_block_of(blockIdx.x, gridDim.x, items, block_item, it, local)) return false;
+  e = &items[it];
+  return true;
 }
 
 template <int NS, int SK, int SS>
 __global__ void __launch_bounds__(256)
 batch_fwd_kernel(const BatchItem* __restrict__ items, const int* __restrict__ block_item,
                  Coeffs<float> cf) {
-  const int b = batch_band_remap(blockIdx.x, gridDim.x);
-  const BatchItem& e = items[block_item[b]];
-  const int local = b - e.block0;
+  const BatchItem* ep;
+  int local;
+  if (!batch_block(items, block_item, ep, local)) return;
+  const BatchItem& e = *ep;
   int s, strip;
   if (!batch_tile(local, e, s, strip)) return;
   const RowsRt rows{{static_cast<const float*>(e.src), e.sp, e.H},
@@ -175,9 +161,10 @@ template <int NS, int SK, int SS>
 __global__ void __launch_bounds__(256)
 batch_inv_kernel(const BatchItem* __restrict__ items, const int* __restrict__ block_item,
                  Coeffs<float> cf) {
-  const int b = batch_band_remap(blockIdx.x, gridDim.x);
-  const BatchItem& e = items[block_item[b]];
-  const int local = b - e.block0;
+  const BatchItem* ep;
+  int local;
+  if (!batch_block(items, block_item, ep, local)) return;
+  const BatchItem& e = *ep;
   int s, strip;
   if (!batch_tile(local, e, s, strip)) return;
   const OutRt out{{static_cast<float*>(e.out), e.op},

@@ -173,8 +173,19 @@ void BatchTables::build(int wavelet_, bool fwd_, int storage_, const dwt_gpu_ima
       e.nstrips = (e.W + core - 1) / core;
       e.ngroups = (e.nstrips + 3) / 4;
       e.nseg = (e.H + lv.S - 1) / lv.S;
+      int nb = batch_item_blocks(e.nstrips, e.nseg);
+      if (nb >= kBatchSegBandMin) {
+        // a large item: whole rounds of the launch order (padding workgroups
+        // before it map to -1; those after its tiles find no tile), dealt to
+        // the XCDs in bands of segment rows
+        const int pad = (kBatchRound - blocks % kBatchRound) % kBatchRound;
+        bmap.insert(bmap.end(), pad, -1);
+        blocks += pad;
+        e.band = batch_item_blocks(e.nstrips, kBatchBandSegs);
+        nb = (nb + kBatchRound - 1) / kBatchRound * kBatchRound;
+      }
+      e.nbp = nb;
       e.block0 = blocks;
-      const int nb = batch_item_blocks(e.nstrips, e.nseg);
       const int idx = (int)(items.size() - lv.items_off);
       bmap.insert(bmap.end(), nb, idx);
       blocks += nb;
@@ -433,6 +444,54 @@ int dwt_gpu_batch_tail_set(long long max_region_bytes) {
   g_batch_tail_bytes = max_region_bytes;
   ++g_batch_config_gen;
   return DWT_GPU_OK;
+}
+
+int dwt_gpu_batch_order_check(int wavelet, int direction, const dwt_gpu_image* images, int n) {
+  try {
+    g_err.clear();
+    batch_check_kind(wavelet, direction, kStoreF32);
+    if (n <= 0 || !images) fail(DWT_GPU_EINVAL, "empty batch");
+    const bool fwd = direction == DWT_GPU_FORWARD;
+    std::vector<dwt_gpu_image> desc(images, images + n);
+    std::vector<char*> base(n);
+    for (int i = 0; i < n; ++i) {
+      batch_check_image(i, desc[i], fwd, kStoreF32);
+      base[i] = reinterpret_cast<char*>(uintptr_t(1) << 40);  // never dereferenced
+    }
+    BatchTables t;
+    t.build(wavelet, fwd, kStoreF32, desc.data(), n, base.data());
+    // every level launch: each tile (segment, strip) of each of its items is
+    // computed by exactly one wave, through the kernels' own mapping
+    for (size_t li = 0; li < t.levels.size(); ++li) {
+      const BatchLevel& lv = t.levels[li];
+      const BatchItem* items = t.items.data() + lv.items_off;
+      const int* bmap = t.bmap.data() + lv.blocks_off;
+      const size_t end = li + 1 < t.levels.size() ? (size_t)t.levels[li + 1].items_off : t.items.size();
+      const int ni = (int)(end - lv.items_off);
+      std::vector<std::vector<char>> seen(ni);
+      for (int k = 0; k < ni; ++k) seen[k].assign((size_t)items[k].nstrips * items[k].nseg, 0);
+      for (int b = 0; b < lv.nblocks; ++b) {
+        int it, local;
+        if (!batch_block_of(b, lv.nblocks, items, bmap, it, local)) continue;
+        if (it < 0 || it >= ni || local < 0 || local >= items[it].nbp)
+          fail(DWT_GPU_EINVAL, "level %d: workgroup %d maps outside its item", lv.level, b);
+        for (int w = 0; w < 4; ++w) {
+          int sg, strip;
+          if (!batch_tile_of(local, w, items[it], sg, strip)) continue;
+          if (sg < 0 || strip < 0 || strip >= items[it].nstrips)
+            fail(DWT_GPU_EINVAL, "level %d: workgroup %d wave %d has a tile outside its item", lv.level, b, w);
+          if (seen[it][(size_t)sg * items[it].nstrips + strip]++)
+            fail(DWT_GPU_EINVAL, "level %d: tile (%d, %d) of item %d computed twice", lv.level, sg, strip, it);
+        }
+      }
+      for (int k = 0; k < ni; ++k)
+        for (size_t q = 0; q < seen[k].size(); ++q)
+          if (!seen[k][q]) fail(DWT_GPU_EINVAL, "level %d: tile %zu of item %d not computed", lv.level, q, k);
+    }
+    return DWT_GPU_OK;
+  } catch (const Failure& f) {
+    return f.code;
+  }
 }
 
 int dwt_gpu_batch_rows_set(int level, int seg_rows) {
