@@ -242,3 +242,43 @@ def test_batch_autotune_installs_a_valid_table():
             np.testing.assert_array_equal(c.cpu().numpy(), ref, err_msg=f"{x.shape} J={j}")
     finally:
         autotune.reset_batch()
+
+
+LARGE_SIZES = [(129, 300), (4096, 4096), (64, 64), (2500, 2100), (1023, 1021), (3001, 2047), (2, 9)]
+
+
+@pytest.mark.parametrize("wavelet", ["cdf97", "cdf53"])
+def test_batch_large_images_match_single_image(wavelet):
+    """Images large enough that their level-0 and level-1 launches take the
+    per-image XCD bands (at least 1024 workgroups: 4096^2, 2500x2100,
+    3001x2047), between small ones: FP32 coefficients and reconstructions
+    bit-identical to the single-image transform of each image (itself
+    bit-identical to the reference, tests/test_gpu_parity.py)."""
+    rng = np.random.default_rng(31)
+    imgs = [rng.standard_normal((H, W)).astype(np.float32) for W, H in LARGE_SIZES]
+    say(f"batch large {wavelet}: {LARGE_SIZES}")
+    ts = [torch.from_numpy(x).cuda() for x in imgs]
+    coefs, _, done = libdwt_amd.batch_forward(ts, wavelet, "f32", -1)
+    outs = libdwt_amd.batch_inverse(coefs, done, wavelet, "f32")
+    torch.cuda.synchronize()
+    for x, c, out, j in zip(ts, coefs, outs, done):
+        H, W = x.shape
+        t = x.clone()
+        jj = libdwt_amd.transform_2d_async(wavelet, True, t.data_ptr(), 4 * W, 4, W, H, j=j)
+        torch.cuda.synchronize()
+        assert jj == j
+        assert torch.equal(c, t), f"forward {W}x{H} J={j}"
+        libdwt_amd.transform_2d_async(wavelet, False, t.data_ptr(), 4 * W, 4, W, H, j=j)
+        torch.cuda.synchronize()
+        assert torch.equal(out, t), f"inverse {W}x{H} J={j}"
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_batch_large_images_16bit(storage):
+    """16-bit storage on the same large-image batch: the rounding, scale and
+    reconstruction bars of check_16bit."""
+    rng = np.random.default_rng(37)
+    imgs = [workloads.sinusoid_image(W, H, 40 + k) if k % 2 else rng.standard_normal((H, W)).astype(np.float32)
+            for k, (W, H) in enumerate(LARGE_SIZES)]
+    say(f"batch large {storage}")
+    check_16bit(imgs, "cdf97", storage)
