@@ -379,9 +379,9 @@ int dwt_gpu_tile_get(int forward, int width, int height, int wavelet, int *seg_r
 
 /* The small levels of a transform run as one single-workgroup launch that
  * keeps the whole small pyramid in LDS, from the first level j >= 1 whose
- * region is at most max_region_bytes (default 64 KiB, at most 76 KiB; 0
+ * region is at most max_region_bytes (default 64 KiB, at most 150 KiB; 0
  * disables it) and whose sides are at most 512 samples.  An image of at most
- * twice that is transformed completely by that one launch. */
+ * twice that (at most 150 KiB) is transformed completely by that one launch. */
 int dwt_gpu_tail_set(long long max_region_bytes);
 
 /* Level 0 of the packed path sends its detail subbands through scratch (one

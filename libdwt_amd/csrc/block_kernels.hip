@@ -2,6 +2,8 @@
 // region held in LDS (the lifting and copies are in block_body.h).
 #include "block_body.h"
 
+#include <algorithm>
+
 namespace dwt {
 
 // Workgroup 0 transforms the region; any further workgroups move rows of a
@@ -119,7 +121,9 @@ block_sparse_inv_kernel(T* __restrict__ a, long long ap, int W0, int H0, int Wi0
 // as the region's rows and columns fit a wavefront's registers (<= 512
 // samples) and the region fits one workgroup's LDS.
 static long long g_block_bytes = 64 * 1024;
-constexpr long long kBlockRegionMax = 76 * 1024;  // twice this (whole images) stays under the LDS limit
+constexpr long long kBlockRegionMax = 150 * 1024;  // a region plus its LDS padding stays under 160 KiB
+// the whole-image limit: twice the threshold, at most the largest region
+static long long whole_image_limit() { return std::min(2 * g_block_bytes, kBlockRegionMax); }
 
 
 void tail_config(long long max_region_bytes, int) {
@@ -136,7 +140,7 @@ int tail_start(int W, int H, int L, int elem) {
   if (g_block_bytes <= 0) return L;
   for (int j = 0; j < L; ++j) {
     const long long wj = ceil_div_pow2(W, j), hj = ceil_div_pow2(H, j);
-    const long long limit = j == 0 ? 2 * g_block_bytes : g_block_bytes;
+    const long long limit = j == 0 ? whole_image_limit() : g_block_bytes;
     if (wj * hj * elem <= limit && wj <= kBlockMaxLine && hj <= kBlockMaxLine &&
         block_lds_bytes(wj, hj, elem) <= kBlockLdsMax)
       return j;
@@ -216,7 +220,7 @@ int sparse_tail_start(int Wo, int Ho, int Wi, int Hi, int L, int elem) {
   for (int j = L - 1; j >= 0; --j) {
     if (ceil_div_pow2(Wi, j) < 2 || ceil_div_pow2(Hi, j) < 2) break;
     const long long wj = ceil_div_pow2(Wo, j), hj = ceil_div_pow2(Ho, j);
-    const long long limit = j == 0 ? 2 * g_block_bytes : g_block_bytes;
+    const long long limit = j == 0 ? whole_image_limit() : g_block_bytes;
     if (wj * hj * elem <= limit && wj <= kBlockMaxLine && hj <= kBlockMaxLine &&
         block_lds_bytes(wj, hj, elem) <= kBlockLdsMax)
       jt = j;

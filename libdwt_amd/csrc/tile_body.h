@@ -20,9 +20,9 @@
 // reference's end-of-line update  x + (2c) * n  bit for bit.
 //
 // Four waves of a workgroup take four adjacent strips of the same segment.
-// Tiles are numbered row-major and the block index is remapped so that each
-// XCD takes a contiguous run of them: a horizontal band of the level, read
-// row after row, whose halo rows and columns hit that XCD's L2.
+// Tiles are numbered row-major and the level kernels remap the block index
+// so that each XCD works on horizontal bands of the level, read row after
+// row, whose halo rows and columns hit that XCD's L2 (tile_kernels.hip).
 //
 // A tile's strips and segments can start at an offset (xoff, yoff: even) so
 // that a fused multi-level kernel computes a window of a level rather than

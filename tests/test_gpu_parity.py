@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 FP32_TOL = 1e-5  # max-abs vs the FP64 oracle on the sinusoid+noise images
 TAIL_DEFAULT = 64 * 1024  # the library's default single-workgroup region
-TAIL_MAX = 76 * 1024      # the largest region dwt_gpu_tail_set accepts
+TAIL_MAX = 150 * 1024     # the largest region dwt_gpu_tail_set accepts
 
 
 def say(msg):
@@ -174,7 +174,7 @@ def test_async_api_on_torch_stream(gold):
 
 
 BLOCK_SHAPES = [(128, 128), (127, 129), (255, 64), (100, 163), (2, 2), (3, 17), (33, 9),
-                (1500, 5), (7, 2000), (90, 91)]
+                (1500, 5), (7, 2000), (90, 91), (190, 190), (376, 256), (301, 511)]
 
 
 @pytest.mark.parametrize("prec", ["s", "d"])
