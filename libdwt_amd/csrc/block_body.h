@@ -245,19 +245,30 @@ __device__ __forceinline__ void block_pass_any(Lds<T>* base, int lstep, int es, 
 // W x H, inner Wi x Hi (src/libdwt.c:6377-6397 forward, 6729-6745 inverse).
 // Forward: every row's samples between its L part and the outer half and past
 // its H part, then every column's likewise.  Inverse: everything past the
-// inner size.  Followed by a barrier in the caller.
+// inner size.  The positions form a few rectangles (full-height column strips
+// and full-width row strips), which the workgroup clears one after another,
+// visiting nothing else; a position in two of them is cleared twice.  Followed
+// by a barrier in the caller.
+template <typename T>
+__device__ __forceinline__ void block_zero_rect(Lds<T>* C, int pc, int x0, int x1, int y0, int y1) {
+  const int w = x1 - x0, h = y1 - y0;
+  if (w <= 0 || h <= 0) return;
+  for (int i = threadIdx.x; i < w * h; i += blockDim.x) {
+    const int y = i / w, x = i - y * w;
+    C[(y0 + y) * pc + x0 + x] = T(0);
+  }
+}
 template <bool FWD, typename T>
 __device__ __forceinline__ void block_zero_pad(Lds<T>* C, int pc, int W, int H, int Wi, int Hi) {
-  const int Wd = (W + 1) >> 1, Hd = (H + 1) >> 1;
-  for (int i = threadIdx.x; i < W * H; i += blockDim.x) {
-    const int y = i / W, x = i - y * W;
-    bool z;
-    if (FWD)
-      z = (x >= ((Wi + 1) >> 1) && x < Wd) || x >= Wd + (Wi >> 1) ||
-          (y >= ((Hi + 1) >> 1) && y < Hd) || y >= Hd + (Hi >> 1);
-    else
-      z = x >= Wi || y >= Hi;
-    if (z) C[y * pc + x] = T(0);
+  if (FWD) {
+    const int Wd = (W + 1) >> 1, Hd = (H + 1) >> 1;
+    block_zero_rect(C, pc, (Wi + 1) >> 1, Wd, 0, H);  // columns between L and the outer half
+    block_zero_rect(C, pc, Wd + (Wi >> 1), W, 0, H);  // columns past H
+    block_zero_rect(C, pc, 0, W, (Hi + 1) >> 1, Hd);  // rows between L and the outer half
+    block_zero_rect(C, pc, 0, W, Hd + (Hi >> 1), H);  // rows past H
+  } else {
+    block_zero_rect(C, pc, Wi, W, 0, H);
+    block_zero_rect(C, pc, 0, W, Hi, H);
   }
 }
 
