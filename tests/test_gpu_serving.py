@@ -40,16 +40,24 @@ def make_requests(n=40, seed=21):
 
 
 def run(server, reqs, one_at_a_time):
+    """Every buffer is made first.  One at a time: submit, then wait for it.
+    Mixed: all requests in one burst from the native replay loop, so that
+    they arrive while earlier ones are on the GPU (a Python loop that
+    uploads each input between submissions can be slower than the server,
+    which then never has more than one request waiting)."""
     outs = []
-    ids = []
     for r in reqs:
         src = torch.from_numpy(r["x"]).cuda()
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()  # inputs ready before the server's streams read them
-        ids.append(server.submit(r["wavelet"], r["fwd"], src, dst, r["J"]))
-        outs.append((src, dst))
-        if one_at_a_time:
+        outs.append((src, torch.empty_like(src)))
+    torch.cuda.synchronize()  # inputs ready before the server's streams read them
+    if one_at_a_time:
+        ids = []
+        for r, (src, dst) in zip(reqs, outs):
+            ids.append(server.submit(r["wavelet"], r["fwd"], src, dst, r["J"]))
             server.drain()
+    else:
+        ids = server.replay([server.request(r["wavelet"], r["fwd"], src, dst, r["J"])
+                             for r, (src, dst) in zip(reqs, outs)], [0.0] * len(reqs))
     server.drain()
     done = {}
     while True:
@@ -103,3 +111,71 @@ def test_rejects_request_larger_than_budget():
     with serving.Server(64 << 10, max_batch=4, streams=1) as s:
         with pytest.raises(libdwt_amd.DwtError):
             s.submit("cdf97", True, x, y, 3)
+
+
+def test_16bit_storage_requests_same_alone_in_the_mix_and_as_a_batch():
+    """FP16 and BF16 requests (forward: image -> 16-bit coefficients and
+    per-subband scales; inverse: back to FP32) mixed with FP32 ones: the
+    coefficients, scales and reconstructions are bit-identical alone and in
+    the mix, and to the batched path's (dwt_gpu_batch_2d) for the same
+    storage, which tests/test_gpu_batch.py holds to its rounding bars."""
+    rng = np.random.default_rng(41)
+    sizes = workloads.uneven_sizes(18, 128, 700, seed=41)
+    storages = ["f16", "bf16", "f32"]
+    reqs = []
+    for k, (W, H) in enumerate(sizes):
+        J = int(rng.integers(1, min(8, workloads.ceil_log2(min(W, H))) + 1))
+        reqs.append(dict(wavelet="cdf97" if k % 4 < 2 else "cdf53", storage=storages[k % 3], J=J,
+                         x=torch.from_numpy(workloads.sinusoid_image(W, H, 300 + k)).cuda()))
+    dtypes = {"f16": torch.float16, "bf16": torch.bfloat16, "f32": torch.float32}
+
+    def submit_all(server, args, one_at_a_time):
+        # buffers made beforehand; mixed: one burst from the native replay loop
+        torch.cuda.synchronize()
+        if one_at_a_time:
+            for a in args:
+                server.submit(*a)
+                server.drain()
+        else:
+            server.replay([server.request(*a) for a in args], [0.0] * len(args))
+        server.drain()
+        while server.poll():
+            pass
+
+    def forward(server, one_at_a_time):
+        outs = [(torch.empty(r["x"].shape, dtype=dtypes[r["storage"]], device="cuda"),
+                 torch.zeros(3 * r["J"] + 1, device="cuda") if r["storage"] != "f32" else None) for r in reqs]
+        submit_all(server, [(r["wavelet"], True, r["x"], c, r["J"], r["storage"], s)
+                            for r, (c, s) in zip(reqs, outs)], one_at_a_time)
+        return outs
+
+    def inverse(server, coefs, one_at_a_time):
+        outs = [torch.empty_like(r["x"]) for r in reqs]
+        submit_all(server, [(r["wavelet"], False, c, y, r["J"], r["storage"], s)
+                            for r, (c, s), y in zip(reqs, coefs, outs)], one_at_a_time)
+        return outs
+
+    with serving.Server(1 << 30, max_batch=64, streams=3) as srv:
+        say("serving 16-bit: forward mixed / alone")
+        mixed = forward(srv, False)
+        alone = forward(srv, True)
+        say("serving 16-bit: inverse mixed / alone")
+        rec_mixed = inverse(srv, mixed, False)
+        rec_alone = inverse(srv, alone, True)
+    for r, (cm, sm), (ca, sa), ym, ya in zip(reqs, mixed, alone, rec_mixed, rec_alone):
+        tag = f"{tuple(r['x'].shape)} {r['wavelet']} {r['storage']} J={r['J']}"
+        assert torch.equal(cm.view(torch.int16) if r["storage"] != "f32" else cm,
+                           ca.view(torch.int16) if r["storage"] != "f32" else ca), tag
+        if sm is not None:
+            assert torch.equal(sm, sa), tag
+        assert torch.equal(ym, ya), tag
+        cb, sb, done = libdwt_amd.batch_forward([r["x"]], r["wavelet"], r["storage"], [r["J"]])
+        yb = libdwt_amd.batch_inverse(cb, done, r["wavelet"], r["storage"], sb)
+        torch.cuda.synchronize()
+        assert done == [r["J"]], tag
+        if r["storage"] != "f32":
+            assert torch.equal(cm.view(torch.int16), cb[0].view(torch.int16)), tag
+            assert torch.equal(sm, sb[0]), tag
+        else:
+            assert torch.equal(cm, cb[0]), tag
+        assert torch.equal(ym, yb[0]), tag
